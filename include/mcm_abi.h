@@ -1,0 +1,211 @@
+/*
+ * mcm_abi.h -- C ABI of libmcm, the MI355X (gfx950) implementation of
+ * MCmatlab's two native hot paths:
+ *
+ *   - voxel Monte Carlo photon transport, replacing the MEX entry
+ *       model = MCmatlab(model, simType)
+ *     (+MCmatlab/src/MCmatlab.c:317 `mexFunction`, field reads :322-615,
+ *      field writes :461-506 and :755-776);
+ *   - heat diffusion + Arrhenius damage, replacing the MEX entry
+ *       [T, Omega, sensorTemps, maxMediaTemps] = finiteElementHeatPropagator(T, Omega, P)
+ *     (+MCmatlab/src/finiteElementHeatPropagator.c:688 `mexFunction`,
+ *      field reads :694-736).
+ *
+ * Conventions
+ *   - Plain C: POD structs, raw pointers, explicit sizes.  No torch or HIP
+ *     types in signatures; HIP streams are passed as `void *` (NULL = the
+ *     library's own stream).
+ *   - Every entry point returns an int status (MCM_OK == 0).  Nothing longjmps
+ *     and nothing spins on a device error (unlike the reference's gpuAssert,
+ *     MCmatlablib.c:165-171).  mcm_last_error() gives the message for the
+ *     calling thread.
+ *   - Arrays are column major exactly as MATLAB hands them to the MEX files
+ *     (x fastest, then y, then z, then wavelength).
+ *   - Input structs mirror the model fields the MEX files read, one member per
+ *     field, in the reference's read order; the comment on each member names
+ *     the MATLAB field and the reference line that reads it.
+ *   - Output buffers are owned by the caller and sized as mcm_mc_sizes()
+ *     reports, except examplePaths, which the library allocates (its length is
+ *     only known after the run) and the caller releases with mcm_free().
+ */
+#ifndef MCM_ABI_H
+#define MCM_ABI_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define MCM_ABI_VERSION 1
+
+/* ---- status codes ---- */
+#define MCM_OK 0
+#define MCM_ERR_INVALID_ARGUMENT (-1)   /* malformed input (sizes, indices, NULLs) */
+#define MCM_ERR_OUT_OF_MEMORY (-2)      /* host or device allocation failed ("MCmatlab:OutOfMemory") */
+#define MCM_ERR_DEVICE (-3)             /* a HIP runtime call failed */
+#define MCM_ERR_NO_DEVICE (-4)          /* no usable gfx950 device */
+#define MCM_ERR_UNSUPPORTED (-5)        /* valid input this build does not implement */
+
+/* Heat propagator numerics (mcm_heat_input.precision / .scheme) */
+#define MCM_HEAT_FP64 0           /* FP64 state on the device (default) */
+#define MCM_HEAT_FP32 1           /* FP32 state, operation order of the reference's FP32 code */
+#define MCM_HEAT_DGADI 0          /* Douglas-Gunn ADI, finiteElementHeatPropagator.c:767-782 (default) */
+#define MCM_HEAT_EXPLICIT 1       /* forward Euler 7-point stencil; stable only for dt <= dtmax */
+
+/* One MCmatlab.sourceIntensityDistribution object (LS.FPID or LS.AID). */
+typedef struct mcm_beam_distr {
+  const double *radialDistr; int64_t n_radialDistr; /* radialDistr, MCmatlab.c:367-368, 539, 548 */
+  double radialWidth;                               /* radialWidth, MCmatlab.c:596, 602 */
+  const double *XDistr; int64_t n_XDistr;           /* XDistr, MCmatlab.c:367-368 */
+  double XWidth;                                    /* XWidth, MCmatlab.c:596, 602 */
+  const double *YDistr; int64_t n_YDistr;           /* YDistr, MCmatlab.c:369-370, 558, 567 */
+  double YWidth;                                    /* YWidth, MCmatlab.c:599, 605 */
+} mcm_beam_distr;
+
+/* Everything MCmatlab.c's mexFunction reads from model.MC (or model.FMC),
+ * model.MC.LS/LC/depositionCriteria and model.G. */
+typedef struct mcm_mc_input {
+  int32_t simType;                    /* prhs[1]: 1 = MC, 2 = FMC (MCmatlab.c:322) */
+  int32_t silentMode;                 /* silentMode (:325) */
+  int32_t calcNFR;                    /* calcNFR (:326) */
+  const float *sourceDistribution;    /* sourceDistribution single [nx,ny,nz,nL], NULL when NaN (:329-330) */
+  double nPhotonsRequested;           /* nPhotonsRequested; NaN = timed run (:333, :337) */
+  int32_t requestCollectedPhotons;    /* requestCollectedPhotons (:334) */
+  double simulationTimeRequested;     /* simulationTimeRequested [min] (:336) */
+  int64_t nM, nL;                     /* size(mediaProperties.mua) (:356-357) */
+  const double *mua, *mus, *g, *n;    /* mediaProperties.{mua,mus,g,n} [nM,nL] (:351-354) */
+  const double *CDFidx;               /* mediaProperties.CDFidx [nM,nL] (:355) */
+  int64_t nx, ny, nz;                 /* size(M) (:362, :380-382) */
+  double dx, dy, dz;                  /* G.dx, G.dy, G.dz (:377-379) */
+  int64_t farFieldRes;                /* farFieldRes (:383) */
+  int32_t boundaryType;               /* boundaryType 0..3 (:384) */
+  const float *interfaceNormals;      /* interfaceNormals single [2,nx*ny*nz] (theta,phi) or NULL (:386) */
+  const uint8_t *M;                   /* M uint8 [nx,ny,nz], 1-based (:387-388) */
+  const double *CDFs; int64_t nCDFs;  /* CDFs double, numel (:390-399) */
+  /* depositionCriteria (:402-416); Inf allowed for the max* and idx fields */
+  double minScatterings, maxScatterings;
+  double minRefractions, maxRefractions;
+  double minReflections, maxReflections;
+  double minInterfaceTransitions, maxInterfaceTransitions;
+  double minSubmediaIdx, maxSubmediaIdx;   /* 1-based */
+  int32_t onlyCollected, evaluateOnlyAtEndOfLife;
+  /* light collector (:427-447) */
+  int32_t useLightCollector;
+  double LC_x, LC_y, LC_z, LC_theta, LC_phi, LC_f, LC_diam;
+  double LC_fieldSize;                /* used when f is finite */
+  double LC_NA;                       /* used when f is infinite */
+  double LC_res, LC_nTimeBins, LC_tStart, LC_tEnd;
+  int64_t nExamplePaths;              /* nExamplePaths (:453) */
+  const double *spectrum;             /* spectrum [nL] (:631) */
+  int32_t useAllCPUs;                 /* useAllCPUs (:727); ignored by the GPU build */
+  int32_t GPUdevice;                  /* GPUdevice (:638) */
+  /* light source LS (:363-366, :579-614) */
+  double sourceType, emitterLength;
+  double LS_theta, LS_phi, LS_psi;
+  double xFocus, yFocus, zFocus;
+  mcm_beam_distr FPID, AID;
+  /* ---- extensions with no reference counterpart ---- */
+  uint64_t seed;                      /* RNG key; the reference seeds from the wall clock (MCmatlab.c:241, 725) */
+  int (*isInterruptPending)(void);    /* polled between launches; NULL = never (reference: utIsInterruptPending) */
+} mcm_mc_input;
+
+/* What MCmatlab.c's mexFunction writes into the duplicated model.  Pointers
+ * for outputs that the input does not request may be NULL. */
+typedef struct mcm_mc_output {
+  float *NFR;            /* NFR single [nx,ny,nz,nL] when calcNFR (:467, normalisation MCmatlablib.c:1204-1207) */
+  float *image;          /* LC.image single [res,res,nTimeBins+2 or 1,nL] when useLightCollector (:469-470) */
+  float *farField;       /* farField single [ffr,ffr,nL] when farFieldRes (:472-474) */
+  float *NI_xpos, *NI_xneg, *NI_ypos, *NI_yneg, *NI_zpos, *NI_zneg;  /* (:476-493) */
+  double *examplePaths;  /* library-allocated [4,examplePathsElems]; release with mcm_free (:772-776) */
+  int64_t examplePathsElems;
+  double nPhotons, nPhotonsCollected, nThreads, simulationTime;  /* (:755-763); simulationTime in minutes */
+} mcm_mc_output;
+
+/* Element counts of every output array for a given input (0 = not produced). */
+typedef struct mcm_mc_sizes {
+  int64_t NFR, image, farField;
+  int64_t NI_xpos, NI_xneg, NI_ypos, NI_yneg, NI_zpos, NI_zneg;
+  int64_t image_dims[4], NI_dims[6][3];
+} mcm_mc_sizes;
+
+/* Everything finiteElementHeatPropagator.c's mexFunction reads. */
+typedef struct mcm_heat_input {
+  int64_t nx, ny, nz;                 /* size(T) (:694-698) */
+  int64_t nM;                         /* size(P.dTdtperdeltaT,1) (:699) */
+  const float *T;                     /* T single [nx,ny,nz] (:702) */
+  const float *Omega;                 /* Omega single [nx,ny,nz], or NULL when Omega is NaN (:722-723) */
+  int64_t steps;                      /* P.steps (:701) */
+  double dt;                          /* P.dt (:707) */
+  const uint8_t *M;                   /* P.M uint8 [nx,ny,nz], 0-based (:708) */
+  const float *dTdtperdeltaT;         /* P.dTdtperdeltaT single [nM,nM,3] (:709) */
+  const float *dTdt_abs;              /* P.dTdt_abs single [nx,ny,nz] (:710) */
+  const double *A, *E;                /* P.A, P.E double [nM] (:711-718) */
+  int32_t lightsOn;                   /* P.lightsOn (:719) */
+  int32_t heatBoundaryType;           /* P.heatBoundaryType: 0 insulating, 1 fixed-temperature boundary voxels (:720) */
+  const double *tempSensorCornerIdxs; /* P.tempSensorCornerIdxs double [nS], 0-based (:728) */
+  int64_t nSensors;                   /* numel(P.tempSensorCornerIdxs) (:729) */
+  const double *tempSensorInterpWeights; /* P.tempSensorInterpWeights double [nS,3] (:730) */
+  int32_t useAllCPUs;                 /* P.useAllCPUs (:755); ignored by the GPU build */
+  /* ---- extensions with no reference counterpart ---- */
+  int32_t precision;                  /* MCM_HEAT_FP64 (default) or MCM_HEAT_FP32 */
+  int32_t scheme;                     /* MCM_HEAT_DGADI (default) or MCM_HEAT_EXPLICIT */
+  const double *T_f64;                /* optional FP64 state replacing T (chained FP64 calls) */
+  const double *Omega_f64;            /* optional FP64 state replacing Omega */
+  int32_t GPUdevice;
+  int (*isInterruptPending)(void);    /* polled between steps; NULL = never */
+} mcm_heat_input;
+
+typedef struct mcm_heat_output {
+  float *T;               /* plhs[0] single [nx,ny,nz] */
+  float *Omega;           /* plhs[1] single [nx,ny,nz], or 1 element set to NaN without damage */
+  double *sensorTemps;    /* plhs[2] double [nS, steps+1] */
+  float *maxMediaTemps;   /* plhs[3] single [1,nM] */
+  double *T_f64;          /* optional FP64 copy of the final state */
+  double *Omega_f64;      /* optional FP64 copy of the final damage integral */
+  int64_t stepsDone;      /* < steps only when interrupted */
+} mcm_heat_output;
+
+/* ---- library ---- */
+const char *mcm_last_error(void);
+int mcm_version(void);
+/* Number of visible gfx950 devices (0 when none); never fails. */
+int mcm_device_count(void);
+void mcm_free(void *p);
+
+/* ---- Monte Carlo ---- */
+int mcm_mc_output_sizes(const mcm_mc_input *in, mcm_mc_sizes *sizes);
+/* Whole MEX-equivalent call: all wavelengths, host buffers in and out. */
+int mcm_mc_run(const mcm_mc_input *in, mcm_mc_output *out);
+
+/* Device-resident form of the same call, for callers that keep the model on
+ * the GPU across many launches (benchmarks, sweeps, gradient loops).
+ * plan_create uploads the grid and tables once; plan_simulate launches
+ * nPhotons packets of wavelength iL into the plan's FP64 tallies (async on
+ * `stream`); plan_normalize waits, normalises like the reference and writes
+ * the wavelength's slice of `out`, then clears the tallies. */
+typedef struct mcm_mc_plan mcm_mc_plan;
+int mcm_mc_plan_create(const mcm_mc_input *in, mcm_mc_plan **plan);
+int mcm_mc_plan_simulate(mcm_mc_plan *plan, int64_t iL, uint64_t nPhotons, uint64_t seed,
+                         uint64_t firstPhoton, void *stream);
+int mcm_mc_plan_normalize(mcm_mc_plan *plan, int64_t iL, mcm_mc_output *out);
+/* Raw FP64 tallies on the device (absorbed weight per voxel, before
+ * normalisation), valid until the next plan call. */
+int mcm_mc_plan_tallies(mcm_mc_plan *plan, const double **d_deposit, uint64_t *nPhotons);
+int mcm_mc_plan_destroy(mcm_mc_plan *plan);
+
+/* ---- heat ---- */
+int mcm_heat_run(const mcm_heat_input *in, mcm_heat_output *out);
+
+typedef struct mcm_heat_plan mcm_heat_plan;
+int mcm_heat_plan_create(const mcm_heat_input *in, mcm_heat_plan **plan);
+/* Advance `steps` steps of size dt with the given illumination; async on the
+ * plan's stream.  Sensors are recorded into the plan's device buffer. */
+int mcm_heat_plan_advance(mcm_heat_plan *plan, int64_t steps, double dt, int32_t lightsOn);
+int mcm_heat_plan_download(mcm_heat_plan *plan, mcm_heat_output *out);
+int mcm_heat_plan_destroy(mcm_heat_plan *plan);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

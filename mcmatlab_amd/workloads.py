@@ -1,0 +1,103 @@
+"""The named workloads: layered skin with a blood vessel on a voxel cube.
+
+Geometry (all sizes in cm, voxel centres as in geometry.m:130-138):
+  air        z < 0.01                      mua 1e-8  mus 1e-8  g 1    n 1
+  epidermis  0.01 <= z < 0.02              mua 20    mus 200   g 0.9  n 1 or 1.4
+  dermis     z >= 0.02                     mua 0.5   mus 200   g 0.9  n 1 or 1.4
+  blood      y^2 + (z - 0.05)^2 < 0.01^2   mua 230   mus 470   g 0.9  n 1 or 1.38
+             (a cylinder along x; overrides the layers)
+
+- layered_tissue_mc(n=100, matched=True):  100^3 pencil-beam run, all n = 1
+  (the CPU-runnable case) -- 1e6 packets by default.
+- layered_tissue_mc(n=200, matched=False, beam="tophat"): the 200^3 case with
+  Fresnel refraction/reflection, a 0.03 cm top-hat focal-plane profile and a
+  Gaussian angular profile of 5 degrees 1/e^2 half-angle -- 1e8 packets.
+- layered_tissue_heat(n=200): the heat run on the same grid: every medium
+  kappa 0.0037 W/(cm K), VHC 3.7 J/(cm^3 K); blood Arrhenius E 422.5 kJ/mol,
+  A 7.6e66 /s; 1 W for 5 ms then 10 ms off as 667 + 1333 steps; 37 C start;
+  fixed-temperature boundary voxels.
+"""
+from __future__ import annotations
+
+import math
+
+import numpy as np
+
+from .media import heat_coefficients, interface_normals, sensor_weights
+from .model import Geometry, Model, SourceIntensityDistribution, media_table
+
+SIDE = 0.1
+MUA = [1e-8, 20.0, 0.5, 230.0]
+MUS = [1e-8, 200.0, 200.0, 470.0]
+G_ANISO = [1.0, 0.9, 0.9, 0.9]
+
+
+def layered_media_map(n, side=SIDE):
+    d = side / n
+    z = (np.arange(n) + 0.5) * d
+    y = (np.arange(n) - (n - 1) / 2) * d
+    M = np.full((n, n, n), 3, np.uint8)
+    M[:, :, z < 0.02] = 2
+    M[:, :, z < 0.01] = 1
+    Y, Z = np.meshgrid(y, z, indexing="ij")
+    M[:, (Y ** 2 + (Z - 0.05) ** 2) < 1e-4] = 4
+    return M
+
+
+def layered_tissue_mc(n=100, matched=True, beam="pencil", nPhotons=1e6, boundaryType=1, side=SIDE):
+    d = side / n
+    m = Model()
+    m.G = Geometry(dx=d, dy=d, dz=d)
+    M = layered_media_map(n, side)
+    ri = [1.0, 1.0, 1.0, 1.0] if matched else [1.0, 1.4, 1.4, 1.38]
+    mc = m.MC
+    mc.M = M
+    mc.mediaProperties = media_table(MUA, MUS, G_ANISO, ri)
+    mc.matchedInterfaces = matched
+    mc.interfaceNormals = np.float32(np.nan) if matched else interface_normals(M, ri, (d, d, d))
+    mc.nPhotonsRequested = float(nPhotons)
+    mc.boundaryType = boundaryType
+    mc.spectrum = np.array([1.0])
+    ls = mc.LS
+    if beam == "pencil":
+        ls.sourceType = 0
+    elif beam == "tophat":
+        ls.sourceType = 4
+        ls.FPID = SourceIntensityDistribution(radialDistr=0, radialWidth=0.03)
+        ls.AID = SourceIntensityDistribution(radialDistr=1, radialWidth=math.radians(5.0))
+    else:
+        raise ValueError(beam)
+    return m
+
+
+def layered_tissue_heat(n=200, side=SIDE, power=1.0, NFR=None, steps_on=667, steps_off=1333,
+                        duration_on=5e-3, duration_off=10e-3):
+    """(T0, Omega0, P_on, P_off) MEX arguments for the heat run.  NFR is the
+    normalized fluence rate of the MC run on the same grid (single
+    [n,n,n]); None uses a synthetic exponential-decay fluence."""
+    d = side / n
+    M = layered_media_map(n, side)
+    nM = 4
+    VHC = np.full(nM, 3.7, np.float32)
+    TC = np.full(nM, 0.0037, np.float32)
+    c = heat_coefficients(VHC, TC, d, d, d)
+    if NFR is None:
+        z = (np.arange(n) + 0.5) * d
+        x = (np.arange(n) - (n - 1) / 2) * d
+        X, Y, Z = np.meshgrid(x, x, z, indexing="ij")
+        NFR = (np.exp(-(X ** 2 + Y ** 2) / (2 * 0.02 ** 2)) * np.exp(-Z / 0.03) / (2 * math.pi * 0.02 ** 2)).astype(np.float32)
+    mua = np.asarray(MUA, np.float32)[M - 1]
+    NA = NFR * mua                                   # normalized absorption [W/cm^3 per W]
+    dTdt_abs = (NA * np.float32(power) / VHC[M - 1]).astype(np.float32)
+    A = np.array([0, 0, 0, 7.6e66])
+    E = np.array([0, 0, 0, 422.5e3])
+    xs = (np.arange(n) - (n - 1) / 2) * d
+    zs = (np.arange(n) + 0.5) * d
+    li, w = sensor_weights([[0, 0, 0.015], [0, 0, 0.05], [0.01, 0.0, 0.03]], xs, xs, zs)
+    base = {"M": (M - 1).astype(np.uint8), "A": A, "E": E, "dTdtperdeltaT": c, "dTdt_abs": dTdt_abs,
+            "useAllCPUs": True, "heatBoundaryType": 1, "tempSensorCornerIdxs": li, "tempSensorInterpWeights": w}
+    P_on = dict(base, lightsOn=True, steps=steps_on, dt=duration_on / steps_on)
+    P_off = dict(base, lightsOn=False, steps=steps_off, dt=duration_off / steps_off)
+    T0 = np.full((n, n, n), 37.0, np.float32)
+    Om0 = np.zeros((n, n, n), np.float32)
+    return T0, Om0, P_on, P_off
