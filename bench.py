@@ -1,0 +1,234 @@
+#!/usr/bin/env python3
+"""Benchmark: Monte Carlo photon packets per second on the 200^3 layered-skin
+workload with Fresnel interfaces and a divergent top-hat beam, on one
+MI355X, through the library's device-resident plan API (the C ABI of
+include/mcm_abi.h).
+
+A step is one full run of the workload: 1e8 packets (FP32 transport, FP64
+fluence) launched as kernel launches of 2^24 packets into the plan's FP64
+tallies; inputs are resident in HBM before timing starts.  Prints one JSON
+line (see the keys below), including:
+  roofline      the transport kernel's algorithmic bytes per launch (packets
+                per launch x 17 E[S] + 72 E[X], E from the oracle's event
+                counts in tests/golden/roofline_counts.json) over its mean
+                launch time from HIP events on the launch stream, against the
+                8 TB/s HBM peak;
+  cpu_baseline  the FP64 CPU restatement (OpenMP, dSFMT) timed on this host's
+                cores on a bounded sample of the same workload.
+  heat          the heat propagator on the same grid (2000 FP64 steps), when
+                the heat kernels are built.
+
+Usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--packets N]
+"""
+from __future__ import annotations
+
+import argparse
+import ctypes as C
+import json
+import math
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+WORKLOAD = "layered_tissue_200_fresnel_tophat"
+HBM_PEAK_GBS = 8000.0
+LAUNCH_PACKETS = 1 << 24
+
+
+class Hip:
+    """The few HIP runtime calls the timing needs (events and a stream)."""
+
+    def __init__(self):
+        self.h = C.CDLL("libamdhip64.so")
+        self.h.hipEventElapsedTime.argtypes = [C.POINTER(C.c_float), C.c_void_p, C.c_void_p]
+        self.h.hipEventRecord.argtypes = [C.c_void_p, C.c_void_p]
+        self.h.hipEventSynchronize.argtypes = [C.c_void_p]
+        self.h.hipStreamSynchronize.argtypes = [C.c_void_p]
+        self.h.hipEventDestroy.argtypes = [C.c_void_p]
+
+    def _ok(self, rc, what):
+        if rc != 0:
+            raise RuntimeError(f"{what} failed with hipError {rc}")
+
+    def stream(self):
+        s = C.c_void_p()
+        self._ok(self.h.hipStreamCreate(C.byref(s)), "hipStreamCreate")
+        return s
+
+    def event(self):
+        e = C.c_void_p()
+        self._ok(self.h.hipEventCreate(C.byref(e)), "hipEventCreate")
+        return e
+
+    def record(self, e, s):
+        self._ok(self.h.hipEventRecord(e, s), "hipEventRecord")
+
+    def elapsed_ms(self, a, b):
+        ms = C.c_float()
+        self._ok(self.h.hipEventSynchronize(b), "hipEventSynchronize")
+        self._ok(self.h.hipEventElapsedTime(C.byref(ms), a, b), "hipEventElapsedTime")
+        return ms.value
+
+    def sync(self, s=None):
+        if s is None:
+            self._ok(self.h.hipDeviceSynchronize(), "hipDeviceSynchronize")
+        else:
+            self._ok(self.h.hipStreamSynchronize(s), "hipStreamSynchronize")
+
+
+def roofline_counts():
+    with open(os.path.join(ROOT, "tests", "golden", "roofline_counts.json")) as f:
+        return json.load(f)["workloads"][WORKLOAD]
+
+
+def cpu_baseline(model, seconds=15.0):
+    """FP64 oracle (OpenMP, dSFMT streams seeded seed+thread) on a bounded
+    sample of the same workload; photons/s on min(16, host cores) threads."""
+    from mcmatlab_amd import abi
+
+    lib = C.CDLL(os.path.join(ROOT, "oracle", "_build", "liboracle_omp.so"))
+    lib.oracle_mc_run.argtypes = [C.POINTER(abi.mcm_mc_input), C.POINTER(abi.mcm_mc_output), C.c_int, C.c_int, C.c_uint64, C.c_void_p]
+    cores = max(1, min(16, os.cpu_count() or 1))
+
+    def run(n):
+        model.MC.nPhotonsRequested = float(n)
+        inp = abi.MCInputs(model, seed=0)
+        out = abi.MCOutputs(inp)
+        t = time.perf_counter()
+        rc = lib.oracle_mc_run(C.byref(inp.struct), C.byref(out.struct), cores, 0, 0, None)
+        dt = time.perf_counter() - t
+        if rc != 0:
+            raise RuntimeError("oracle failed")
+        return out.struct.nPhotons, dt
+
+    n0, dt0 = run(1000 * cores)
+    n = max(2000, int(n0 / max(dt0, 1e-3) * seconds))
+    n1, dt1 = run(n)
+    return {"value": n1 / dt1, "unit": "photons/s", "cores": cores, "kind": "port",
+            "sample": f"{int(n1)} packets of the {WORKLOAD} workload, FP64 transport, dSFMT, OpenMP ({dt1:.1f} s)"}
+
+
+def bench_heat(hip, steps_on=667, steps_off=1333):
+    """Config-2 heat schedule on the same 200^3 grid through the heat plan;
+    returns voxel-steps/s or None when the heat kernels are not built."""
+    from mcmatlab_amd import abi
+    from mcmatlab_amd._lib import check, lib
+    from mcmatlab_amd.workloads import layered_tissue_heat
+
+    T0, Om0, Pon, Poff = layered_tissue_heat(n=200, steps_on=steps_on, steps_off=steps_off)
+    inp = abi.HeatInputs(T0, Om0, Pon)
+    h = C.c_void_p()
+    check(lib().mcm_heat_plan_create(C.byref(inp.struct), C.byref(h)))
+    try:
+        # warm-up: 10 steps
+        check(lib().mcm_heat_plan_advance(h, 10, Pon["dt"], 1))
+        hip.sync()
+        t = time.perf_counter()
+        check(lib().mcm_heat_plan_advance(h, steps_on, Pon["dt"], 1))
+        check(lib().mcm_heat_plan_advance(h, steps_off, Poff["dt"], 0))
+        hip.sync()
+        dt = time.perf_counter() - t
+    finally:
+        lib().mcm_heat_plan_destroy(h)
+    L = 200 ** 3
+    return {"voxel_steps_per_s": L * (steps_on + steps_off) / dt, "steps": steps_on + steps_off, "seconds": dt,
+            "scheme": "DG-ADI", "dtype": "f64",
+            "algorithmic_GBs": L * (steps_on * 41 + steps_off * 33) / dt / 1e9}
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=3)
+    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--packets", type=float, default=1e8)
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-heat", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
+        print("bench.py measures one GPU only: this build has no multi-GPU path (run with --gpus 1)", file=sys.stderr)
+        sys.exit(2)
+
+    from mcmatlab_amd.api import MCPlan
+    from mcmatlab_amd.workloads import layered_tissue_mc
+
+    t_setup = time.perf_counter()
+    model = layered_tissue_mc(n=200, matched=False, beam="tophat", nPhotons=args.packets)
+    hip = Hip()
+    stream = hip.stream()
+    plan = MCPlan(model)
+    print(f"[bench] setup {time.perf_counter() - t_setup:.1f}s", file=sys.stderr, flush=True)
+    npk = int(args.packets)
+    launches = []
+    first = 0
+
+    def step(record):
+        nonlocal first
+        done = 0
+        while done < npk:
+            n = min(LAUNCH_PACKETS, npk - done)
+            if record:
+                e0, e1 = hip.event(), hip.event()
+                hip.record(e0, stream)
+            plan.simulate(n, seed=12345, firstPhoton=first, stream=stream)
+            if record:
+                hip.record(e1, stream)
+                launches.append((e0, e1, n))
+            first += n
+            done += n
+
+    for _ in range(args.warmup):
+        step(False)
+    hip.sync()
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        step(True)
+    hip.sync()
+    t1 = time.perf_counter()
+    elapsed = t1 - t0
+    value = args.steps * npk / elapsed
+
+    durs = [(hip.elapsed_ms(a, b), n) for a, b, n in launches]
+    full = [d for d, n in durs if n == LAUNCH_PACKETS] or [d for d, _ in durs]
+    n_per = LAUNCH_PACKETS if any(n == LAUNCH_PACKETS for _, n in durs) else durs[0][1]
+    avg_ms = sum(full) / len(full)
+    rc = roofline_counts()
+    alg_bytes = rc["bytes_per_packet"] * n_per
+    achieved = alg_bytes / (avg_ms / 1e3) / 1e9
+    # sanity of the physics on the timed tallies
+    res = plan.normalize()
+    import numpy as np
+
+    mua = np.asarray(model.MC.mediaProperties["mua"])[:, 0]
+    absorbed = float(np.sum(np.asarray(res["NFR"], dtype=np.float64)[..., 0] * mua[model.MC.M - 1]) * model.G.dx ** 3)
+    plan.close()
+
+    line = {
+        "metric": "photons/s at 200^3 tissue (Fresnel, top-hat beam), FP32 transport with FP64 fluence",
+        "value": value, "unit": "photons/s", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
+        "ms_per_step": elapsed / args.steps * 1e3, "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+        "dtype": "f32", "data": "synthetic",
+        "config": {"workload": WORKLOAD, "grid": [200, 200, 200], "packets_per_step": npk, "fresnel": True,
+                   "beam": "top-hat r=0.03 cm, Gaussian 5 deg divergence", "boundaryType": 1,
+                   "absorbed_fraction": absorbed},
+        "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": achieved / HBM_PEAK_GBS, "traffic": None,
+                     "kernel": "mc_transport", "launch_ms": avg_ms, "packets_per_launch": n_per,
+                     "bytes_per_packet": rc["bytes_per_packet"]},
+    }
+    if not args.no_heat:
+        try:
+            line["heat"] = bench_heat(hip)
+        except Exception as e:  # heat kernels not available in this build
+            line["heat"] = {"error": str(e)[:200]}
+    if not args.no_cpu:
+        line["cpu_baseline"] = cpu_baseline(layered_tissue_mc(n=200, matched=False, beam="tophat", nPhotons=1000))
+        line["vs_cpu"] = value / line["cpu_baseline"]["value"]
+    print(json.dumps(line), flush=True)
+
+
+if __name__ == "__main__":
+    main()

@@ -1,0 +1,124 @@
+"""The two operator entry points, with the reference MEX files' names,
+arguments and outputs:
+
+  model = MCmatlab(model, simType)
+      (+MCmatlab/src/MCmatlab.c:317; called by @model/runMonteCarlo.m:99-104,
+      :168-175) -- voxel Monte Carlo on the GPU; returns a copy of the model
+      with MC (simType 1) or FMC (simType 2) outputs filled in: NFR,
+      farField, NI_xpos ... NI_zneg, LC.image, examplePaths, nPhotons,
+      nPhotonsCollected, nThreads, simulationTime.
+
+  T, Omega, sensorTemps, maxMediaTemps = finiteElementHeatPropagator(T, Omega, P)
+      (+MCmatlab/src/finiteElementHeatPropagator.c:688; called by
+      @model/simulateHeatDistribution.m:259-264) -- `steps` heat steps of
+      size dt on the GPU.
+
+Errors raise MCMError (the MEX files raise MATLAB errors via
+mexErrMsgIdAndTxt).  There is no CPU fallback.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import time
+
+import numpy as np
+
+from . import abi
+from ._lib import check, lib
+
+_OUT_FIELDS = ("NFR", "farField", "NI_xpos", "NI_xneg", "NI_ypos", "NI_yneg", "NI_zpos", "NI_zneg")
+
+
+def _squeeze_trailing(a):
+    """MATLAB drops trailing singleton dimensions beyond the second."""
+    shape = list(a.shape)
+    while len(shape) > 2 and shape[-1] == 1:
+        shape.pop()
+    return a.reshape(shape, order="F")
+
+
+def MCmatlab(model, simType=1, seed=None, interrupt=None):
+    """model = MCmatlab(model, simType).  `seed` keys the packet streams
+    (the reference seeds from the clock; default: the clock)."""
+    if simType not in (1, 2):
+        raise ValueError("simType must be 1 (MC) or 2 (fluorescence MC)")
+    if seed is None:
+        seed = time.time_ns() & 0xFFFFFFFFFFFFFFFF
+    out_model = model.copy()
+    inp = abi.MCInputs(model, simType=simType, seed=seed, interrupt=interrupt)
+    sizes = abi.mcm_mc_sizes()
+    check(lib().mcm_mc_output_sizes(C.byref(inp.struct), C.byref(sizes)))
+    outs = abi.MCOutputs(inp)
+    check(lib().mcm_mc_run(C.byref(inp.struct), C.byref(outs.struct)))
+    res = outs.result(free_fn=lambda p: lib().mcm_free(C.cast(p, C.c_void_p)))
+    mc = out_model.FMC if simType == 2 else out_model.MC
+    for k in _OUT_FIELDS:
+        if k in res:
+            setattr(mc, k, _squeeze_trailing(res[k]))
+    if "image" in res:
+        img = _squeeze_trailing(res["image"])
+        mc.LC.image = img.reshape(()) if img.size == 1 else img
+    if "examplePaths" in res:
+        mc.examplePaths = res["examplePaths"]
+    for k in ("nPhotons", "nPhotonsCollected", "nThreads", "simulationTime"):
+        setattr(mc, k, res[k])
+    return out_model
+
+
+def finiteElementHeatPropagator(T, Omega, P, precision="fp64", scheme="dgadi", state_f64=None, interrupt=None):
+    """[T, Omega, sensorTemps, maxMediaTemps] = finiteElementHeatPropagator(T, Omega, P).
+
+    precision: 'fp64' (device state in FP64, default) or 'fp32' (the
+    reference's FP32 arithmetic).  scheme: 'dgadi' (the reference's
+    Douglas-Gunn ADI, default) or 'explicit' (forward Euler; stable only for
+    P['dt'] <= dtmax).  state_f64=(T64, Omega64) continues an FP64 state from
+    a previous call; the FP64 final state is returned in the dict
+    `finiteElementHeatPropagator.last_state`."""
+    prec = {"fp64": abi.MCM_HEAT_FP64, "fp32": abi.MCM_HEAT_FP32}[precision]
+    sch = {"dgadi": abi.MCM_HEAT_DGADI, "explicit": abi.MCM_HEAT_EXPLICIT}[scheme]
+    T64, Om64 = (state_f64 if state_f64 is not None else (None, None))
+    inp = abi.HeatInputs(T, Omega, P, precision=prec, scheme=sch, T_f64=T64, Omega_f64=Om64, interrupt=interrupt)
+    out = abi.HeatOutputs(inp, want_f64=True)
+    check(lib().mcm_heat_run(C.byref(inp.struct), C.byref(out.struct)))
+    r = out.result()
+    finiteElementHeatPropagator.last_state = (r.get("T_f64"), r.get("Omega_f64"))
+    return r["T"], r["Omega"], r["sensorTemps"], r["maxMediaTemps"]
+
+
+finiteElementHeatPropagator.last_state = (None, None)
+
+
+class MCPlan:
+    """Device-resident Monte Carlo plan (mcm_mc_plan_*): upload once, launch
+    many times.  Used by the benchmark and by sweeps."""
+
+    def __init__(self, model, simType=1):
+        self._inp = abi.MCInputs(model, simType=simType, seed=0)
+        h = C.c_void_p()
+        check(lib().mcm_mc_plan_create(C.byref(self._inp.struct), C.byref(h)))
+        self.h = h
+
+    def simulate(self, nPhotons, seed=0, iL=0, firstPhoton=0, stream=None):
+        check(lib().mcm_mc_plan_simulate(self.h, iL, int(nPhotons), int(seed), int(firstPhoton), stream))
+
+    def normalize(self, iL=0):
+        outs = abi.MCOutputs(self._inp)
+        check(lib().mcm_mc_plan_normalize(self.h, iL, C.byref(outs.struct)))
+        return outs.result()
+
+    def tallies(self):
+        p = C.c_void_p()
+        n = C.c_uint64()
+        check(lib().mcm_mc_plan_tallies(self.h, C.byref(p), C.byref(n)))
+        return p.value, n.value
+
+    def close(self):
+        if self.h:
+            lib().mcm_mc_plan_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass

@@ -1,0 +1,140 @@
+// Composition of the photon physics into the per-packet state machine that
+// the GPU kernel schedules across a wavefront (and the host debug harness
+// runs packet by packet).  The state after each step tells the scheduler
+// what the packet needs next:
+//   FETCH    -> the packet died; the lane needs a new packet number
+//   MOVE     -> one voxel sub-step (propagatePhoton's movement + absorption)
+//   SCATTER  -> roulette + scattering (MCmatlab.c:259-260)
+//   FRESNEL  -> reflection/refraction at a refractive-index change
+// Tally writes go through Sink:
+//   deposit(j, w)   absorbed weight into voxel j (the sink may merge
+//                   consecutive deposits into one voxel), flush() at death
+//   image(bin, w), farField(bin, w), edge(face, idx, w), collected()
+#pragma once
+#include "mc_photon.h"
+
+namespace mcm {
+
+enum : int { ST_FETCH = 0, ST_MOVE = 1, ST_SCATTER = 2, ST_FRESNEL = 3, ST_DONE = 4 };
+
+template <bool FRESNEL, bool CRITERIA>
+struct PacketEngine {
+  Photon P;
+  PacketRng R;
+  uint64_t packet;
+  int state;
+  bool replay;
+  float replayW;
+  float pendAbsorb;
+  int pendJ, pendM;
+
+  __host__ __device__ void init(uint32_t k0, uint32_t k1, uint32_t stream) {
+    R.key0 = k0;
+    R.key1 = k1;
+    R.stream = stream;
+    state = ST_FETCH;
+    replay = false;
+    replayW = 0.f;
+    packet = 0;
+  }
+
+  // launchPhoton + getNewVoxelProperties for packet `id` (MCmatlab.c:247-248).
+  // False when the launch fails 10^6 times (the reference aborts the run).
+  __host__ __device__ bool start(uint64_t id, const McParams& p, const Medium* med, const uint8_t* cdf) {
+    packet = id;
+    R.start(id);
+    if (!launch(P, p, med, R)) {
+      state = ST_FETCH;
+      return false;
+    }
+    new_voxel_properties(P, p, med, cdf);
+    state = P.stepLeft > 0.f ? ST_MOVE : ST_SCATTER;
+    return true;
+  }
+
+  template <class Sink>
+  __host__ __device__ void death(const McParams& p, const Medium* med, const uint8_t* cdf, Sink& sink) {
+    sink.flush();
+    if (CRITERIA && !replay && p.evalAtEnd && criteria_met(P, p)) {
+      // end-of-life deposition (MCmatlab.c:262-264): replay the packet and
+      // deposit its absorbed weights scaled by the final weight
+      replay = true;
+      replayW = P.weight;
+      start(packet, p, med, cdf);
+    } else {
+      replay = false;
+      state = ST_FETCH;
+    }
+  }
+
+  // deposit of the sub-step, then checkEscape + getNewVoxelProperties
+  // (propagatePhoton :1095-1110, MCmatlab.c:254-257)
+  template <class Sink>
+  __host__ __device__ void finish_substep(float absorb, const McParams& p, const Medium* med, const uint8_t* cdf, Sink& sink) {
+    if (P.insideVolume && p.calcNFR) {
+      if (CRITERIA && replay) sink.deposit(P.j, (double)(replayW * absorb));
+      else if (!(CRITERIA && p.evalAtEnd) && criteria_met(P, p)) sink.deposit(P.j, (double)absorb);
+    }
+    if (!P.sameVoxel) {
+      const EscapeResult e = check_escape(P, p);
+      if (!replay) {
+        if (e.escaped) {
+          if (p.useLC) {
+            const int b = collector_bin(P, p);
+            if (b >= 0 && criteria_met(P, p)) {
+              sink.image(b, (double)P.weight);
+              sink.collected();
+            }
+          }
+          if (p.farFieldRes && criteria_met(P, p)) sink.farField(far_field_bin(P, p), (double)P.weight);
+        }
+        if (e.edgeFlux && criteria_met(P, p)) {
+          int face;
+          const int idx = edge_flux_bin(P, p, &face);
+          if (face >= 0) sink.edge(face, idx, (double)P.weight);
+        }
+      }
+      if (P.alive) new_voxel_properties(P, p, med, cdf);
+    }
+    if (!P.alive) death(p, med, cdf, sink);
+    else state = P.stepLeft > 0.f ? ST_MOVE : ST_SCATTER;
+  }
+
+  template <class Sink>
+  __host__ __device__ void do_move(const McParams& p, const Medium* med, const uint8_t* cdf, Sink& sink) {
+    if (P.substeps >= kMaxSubsteps) {
+      sink.runaway();
+      P.alive = false;
+      death(p, med, cdf, sink);
+      return;
+    }
+    const MoveResult mr = move<FRESNEL>(P, p, med);
+    if (FRESNEL && mr.fresnel) {
+      pendAbsorb = mr.absorb;
+      pendJ = mr.jNew;
+      pendM = mr.mNew;
+      state = ST_FRESNEL;
+    } else {
+      finish_substep(mr.absorb, p, med, cdf, sink);
+    }
+  }
+
+  template <class Sink>
+  __host__ __device__ void do_fresnel(const McParams& p, const Medium* med, const uint8_t* cdf, Sink& sink) {
+    fresnel(P, p, med, pendJ, pendM, R);
+    finish_substep(pendAbsorb, p, med, cdf, sink);
+  }
+
+  template <class Sink>
+  __host__ __device__ void do_scatter(const McParams& p, const Medium* med, const uint8_t* cdf, Sink& sink) {
+    roulette(P, R);
+    if (!P.alive) {
+      death(p, med, cdf, sink);
+      return;
+    }
+    scatter(P, p, p.cdfs, CRITERIA ? (int)p.M[P.j] : 0, R);
+    state = P.stepLeft > 0.f ? ST_MOVE : ST_SCATTER;
+  }
+};
+
+}  // namespace mcm

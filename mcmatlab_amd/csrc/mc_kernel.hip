@@ -1,0 +1,673 @@
+// Monte Carlo photon transport on gfx950: kernels and the C ABI
+// (mcm_mc_plan_*, mcm_mc_run).  Replaces the reference's photon loop
+// (+MCmatlab/src/MCmatlab.c:178-315) and MEX body (:317-791).
+//
+// Kernel design
+//   One packet per lane, persistent lanes: a lane whose packet dies fetches
+//   the next packet number from a device counter (one atomic per wavefront
+//   fetch) and keeps going until the launch's packets are used up.  Every
+//   packet draws from its own Philox stream, so the set of simulated packets
+//   and each packet's trajectory are independent of scheduling.
+//
+//   Each packet is a small state machine: FETCH (launch), MOVE (one voxel
+//   sub-step with continuous absorption), SCATTER (roulette + deflection) and
+//   FRESNEL (reflection/refraction at a refractive-index change).  Lanes of a
+//   wavefront are usually in different states; instead of executing every
+//   state's code under a divergent mask each iteration, the wavefront votes
+//   (ballot) and runs one state per iteration: MOVE while at least `moveFrac`
+//   of the live lanes want to move, otherwise whichever state has the most
+//   lanes waiting.  Lanes waiting for a rarer state idle briefly and are then
+//   served together, which keeps SIMD occupancy high for the expensive
+//   scatter and interface code.
+//
+//   Per-medium optical constants live in LDS.  Deposits accumulate in a
+//   per-lane FP64 register while the packet stays in one voxel and go to the
+//   FP64 fluence grid with a hardware FP64 atomic add when it leaves (or
+//   dies).  Boundary, far-field and detector tallies are FP64 atomics too.
+//
+//   Deposition criteria evaluated at end of life (MCmatlab.c:262-264) are
+//   handled by replay instead of per-packet deposit records: when a packet
+//   dies meeting the criteria, the lane re-launches the same packet number,
+//   whose stream reproduces the trajectory exactly, and deposits the
+//   absorbed weights scaled by the final weight.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <chrono>
+#include <cmath>
+#include <cstdlib>
+#include <cstring>
+#include <memory>
+#include <vector>
+
+#include "mc_engine.h"
+#include "mc_photon.h"
+#include "mc_setup.h"
+#include "mcm_common.h"
+
+namespace mcm {
+
+struct McCounters {
+  unsigned long long nextPacket;
+  unsigned long long launched;
+  unsigned long long collected;
+  unsigned long long substeps;
+  unsigned long long scatters;
+  unsigned long long interfaces;
+  unsigned int aborted;
+  unsigned int runaway;
+};
+
+struct McLaunch {
+  uint64_t firstPacket;
+  uint64_t nPackets;
+  uint32_t key0, key1, stream;
+  float moveFrac;
+  McCounters* ctr;
+};
+
+__device__ inline unsigned long long bcast64(unsigned long long v, int src) {
+  const unsigned lo = __shfl((unsigned)v, src), hi = __shfl((unsigned)(v >> 32), src);
+  return ((unsigned long long)hi << 32) | lo;
+}
+
+__device__ inline void fadd(double* p, double v) { unsafeAtomicAdd(p, v); }
+
+// Tally sink of the GPU kernel: FP64 hardware atomics; consecutive deposits
+// into the same voxel are merged in a register first.
+struct DeviceSink {
+  const McParams* p;
+  double acc;
+  int accJ;
+  unsigned long long nCollected;
+  unsigned nRunaway;
+  __device__ void deposit(int j, double v) {
+    if (j == accJ) {
+      acc += v;
+    } else {
+      if (accJ >= 0) fadd(&p->NFR[accJ], acc);
+      accJ = j;
+      acc = v;
+    }
+  }
+  __device__ void flush() {
+    if (accJ >= 0) fadd(&p->NFR[accJ], acc);
+    accJ = -1;
+    acc = 0.0;
+  }
+  __device__ void image(int b, double w) { fadd(&p->image[b], w); }
+  __device__ void farField(int b, double w) { fadd(&p->FF[b], w); }
+  __device__ void edge(int f, int i, double w) { fadd(&p->NI[f][i], w); }
+  __device__ void collected() { ++nCollected; }
+  __device__ void runaway() { ++nRunaway; }
+};
+
+template <bool FRESNEL, bool CRITERIA>
+__global__ void __launch_bounds__(256) mc_transport(McParams p, McLaunch lp) {
+  __shared__ Medium sMed[256];
+  __shared__ uint8_t sCdf[256];
+  for (int k = threadIdx.x; k < 256; k += blockDim.x) {
+    sMed[k] = p.media[k];
+    sCdf[k] = p.cdfIdx[k];
+  }
+  __syncthreads();
+  if (!CRITERIA) p.criteriaActive = 0;
+
+  const int lane = __lane_id();
+  PacketEngine<FRESNEL, CRITERIA> E;
+  E.init(lp.key0, lp.key1, lp.stream);
+  DeviceSink sink{&p, 0.0, -1, 0ull, 0u};
+  unsigned long long nLaunched = 0, nSub = 0, nScat = 0, nIf = 0;
+
+  for (;;) {
+    const int st = E.state;
+    const unsigned long long bFetch = __ballot(st == ST_FETCH);
+    const unsigned long long bMove = __ballot(st == ST_MOVE);
+    const unsigned long long bScat = __ballot(st == ST_SCATTER);
+    const unsigned long long bFres = FRESNEL ? __ballot(st == ST_FRESNEL) : 0ull;
+    const int cF = __popcll(bFetch), cM = __popcll(bMove), cS = __popcll(bScat), cR = __popcll(bFres);
+    const int active = cF + cM + cS + cR;
+    if (active == 0) break;
+    // one state per iteration, chosen by the wavefront (see file comment)
+    int phase = ST_MOVE;
+    if ((float)cM < lp.moveFrac * (float)active) {
+      int best = cM;
+      if (cS > best) { best = cS; phase = ST_SCATTER; }
+      if (FRESNEL && cR > best) { best = cR; phase = ST_FRESNEL; }
+      if (cF > best) phase = ST_FETCH;
+    }
+    if (phase == ST_MOVE) {
+      if (st == ST_MOVE) {
+        ++nSub;
+        E.do_move(p, sMed, sCdf, sink);
+      }
+    } else if (phase == ST_SCATTER) {
+      if (st == ST_SCATTER) {
+        ++nScat;
+        E.do_scatter(p, sMed, sCdf, sink);
+      }
+    } else if (FRESNEL && phase == ST_FRESNEL) {
+      if (st == ST_FRESNEL) {
+        ++nIf;
+        E.do_fresnel(p, sMed, sCdf, sink);
+      }
+    } else {
+      // consecutive packet numbers for the waiting lanes, one atomic per wavefront
+      const int leader = __ffsll((long long)bFetch) - 1;
+      unsigned long long base = 0;
+      if (lane == leader) {
+        base = atomicAdd(&lp.ctr->nextPacket, (unsigned long long)cF);
+        if (atomicAdd(&lp.ctr->aborted, 0u)) base = lp.nPackets;  // a failed launch stops the run
+      }
+      base = bcast64(base, leader);
+      if (st == ST_FETCH) {
+        const unsigned long long k = base + (unsigned long long)__popcll(bFetch & ((1ull << lane) - 1ull));
+        if (k >= lp.nPackets) {
+          E.state = ST_DONE;
+        } else {
+          E.replay = false;
+          if (E.start(lp.firstPacket + k, p, sMed, sCdf)) {
+            ++nLaunched;
+          } else {
+            atomicOr(&lp.ctr->aborted, 1u);
+            E.state = ST_DONE;
+          }
+        }
+      }
+    }
+  }
+  sink.flush();
+  if (nLaunched) atomicAdd(&lp.ctr->launched, nLaunched);
+  if (sink.nCollected) atomicAdd(&lp.ctr->collected, sink.nCollected);
+  if (nSub) atomicAdd(&lp.ctr->substeps, nSub);
+  if (nScat) atomicAdd(&lp.ctr->scatters, nScat);
+  if (nIf) atomicAdd(&lp.ctr->interfaces, nIf);
+  if (sink.nRunaway) atomicAdd(&lp.ctr->runaway, (unsigned)sink.nRunaway);
+}
+
+// NFR normalisation (normalizeDepositionAndResetO, MCmatlablib.c:1204-1207):
+// out[j] = dep[j] / (V * normfactor * mua[M[j]]) as single; clears dep.
+__global__ void mc_normalize_nfr(double* dep, const uint8_t* M, const double* mua, double scale, float* out, int64_t L) {
+  const int64_t j = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (j >= L) return;
+  out[j] = (float)(dep[j] / (scale * mua[M[j]]));
+  dep[j] = 0.0;
+}
+
+__global__ void mc_fill_normal_ri(float4* nri, const uint8_t* M, const Medium* media, int64_t L) {
+  const int64_t j = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (j >= L) return;
+  nri[j].w = media[M[j]].ri;
+}
+
+// Example paths (updatePaths/addToPhotonPath/deletePhotonPath,
+// MCmatlablib.c:465-514, and the master-thread bookkeeping of
+// MCmatlab.c:266-279): one thread replays packets 0, 1, 2, ... of the
+// wavelength, recording (x, y, z, weight) after every launch, sub-step and
+// scattering, until nPaths packets have finished a recorded path.  Paths are
+// separated by NaN columns as in the reference.  Writes at most `cap` columns;
+// *nOut receives the number needed.
+__global__ void mc_example_paths(McParams p, McLaunch lp, int64_t nPaths, double* out, int64_t cap, int64_t* nOut) {
+  if (threadIdx.x != 0 || blockIdx.x != 0) return;
+  __shared__ Medium sMed[256];
+  __shared__ uint8_t sCdf[256];
+  for (int k = 0; k < 256; ++k) {
+    sMed[k] = p.media[k];
+    sCdf[k] = p.cdfIdx[k];
+  }
+  int64_t n = 0, finished = 0;
+  PacketRng R;
+  R.key0 = lp.key0;
+  R.key1 = lp.key1;
+  R.stream = lp.stream;
+  Photon P;
+  const float hx = p.n[0] * 0.5f, hy = p.n[1] * 0.5f;
+  bool started = false;
+  auto put = [&](double x, double y, double z, double w) {
+    if (n < cap) {
+      out[4 * n] = x;
+      out[4 * n + 1] = y;
+      out[4 * n + 2] = z;
+      out[4 * n + 3] = w;
+    }
+    ++n;
+  };
+  auto record = [&](bool teleported) {
+    if (!(p.evalAtEnd || criteria_met(P, p))) return;
+    const int nan = started ? (teleported ? 1 : 0) : 2;
+    for (int k = 0; k < nan; ++k) put(NAN, NAN, NAN, NAN);
+    put((P.i[0] - hx) * p.d[0], (P.i[1] - hy) * p.d[1], P.i[2] * p.d[2], P.weight);
+    started = true;
+  };
+  for (uint64_t k = 0; k < lp.nPackets && finished < nPaths; ++k) {
+    R.start(lp.firstPacket + k);
+    started = false;
+    if (!launch(P, p, sMed, R)) break;
+    record(false);
+    new_voxel_properties(P, p, sMed, sCdf);
+    while (P.alive) {
+      while (P.alive && P.stepLeft > 0.f) {
+        if (P.substeps >= kMaxSubsteps) {
+          P.alive = false;
+          break;
+        }
+        const MoveResult mr = move<true>(P, p, sMed);
+        if (mr.fresnel) fresnel(P, p, sMed, mr.jNew, mr.mNew, R);
+        record(false);
+        if (!P.sameVoxel) {
+          const float i0 = P.i[0], i1 = P.i[1];
+          check_escape(P, p);
+          if (p.boundaryType == 3 && (P.i[0] != i0 || P.i[1] != i1)) record(true);
+          if (P.alive) new_voxel_properties(P, p, sMed, sCdf);
+        }
+      }
+      if (P.alive) roulette(P, R);
+      if (P.alive) {
+        scatter(P, p, p.cdfs, p.M[P.j], R);
+        record(false);
+      }
+    }
+    if (started) {
+      if (p.evalAtEnd && !criteria_met(P, p)) {
+        // drop the unfinished path back to its leading NaN pair
+        for (; n > 1; --n) {
+          if (n - 1 < cap && n - 2 < cap && isnan(out[4 * (n - 1)]) && isnan(out[4 * (n - 2)])) {
+            n -= 2;
+            break;
+          }
+        }
+      } else {
+        ++finished;
+      }
+    }
+  }
+  *nOut = n;
+}
+
+}  // namespace mcm
+
+using namespace mcm;
+
+struct DevBuf {
+  void* p = nullptr;
+  ~DevBuf() {
+    if (p) (void)hipFree(p);
+  }
+  template <typename T>
+  T* as() const { return (T*)p; }
+};
+
+struct mcm_mc_plan {
+  int device = 0;
+  hipStream_t stream = nullptr;
+  McTables T;
+  McParams prm;
+  int blocks = 0, threads = 256;
+  float moveFrac = 0.5f;
+  DevBuf M, normals, media, cdfIdx, cdfs, fpid1, aid1, fpid2, aid2, S, muaNorm;
+  DevBuf NFR, image, FF, NI[6], counters, nfrOut;
+  std::vector<double> Shost;   // cumulative 3D source table of the current wavelength
+  const float* sourceDistribution = nullptr;
+  double power = 0;            // current wavelength's source power
+  int curL = -1;
+  uint64_t lastLaunched = 0, lastCollected = 0;
+  McCounters stats{};
+  ~mcm_mc_plan() {
+    if (stream) (void)hipStreamDestroy(stream);
+  }
+};
+
+namespace {
+
+template <typename T>
+int upload(DevBuf& b, const T* src, size_t n, hipStream_t s) {
+  if (b.p) {
+    MCM_HIP(hipFree(b.p));
+    b.p = nullptr;
+  }
+  MCM_HIP(hipMalloc(&b.p, std::max<size_t>(n, 1) * sizeof(T)));
+  if (src && n) MCM_HIP(hipMemcpyAsync(b.p, src, n * sizeof(T), hipMemcpyHostToDevice, s));
+  return MCM_OK;
+}
+
+int zeros(DevBuf& b, size_t bytes, hipStream_t s) {
+  if (b.p) {
+    MCM_HIP(hipFree(b.p));
+    b.p = nullptr;
+  }
+  MCM_HIP(hipMalloc(&b.p, std::max<size_t>(bytes, 8)));
+  MCM_HIP(hipMemsetAsync(b.p, 0, std::max<size_t>(bytes, 8), s));
+  return MCM_OK;
+}
+
+// Select medium table, source table and refractive indices of wavelength iL.
+int select_wavelength(mcm_mc_plan* pl, int64_t iL) {
+  if (pl->curL == iL) return MCM_OK;
+  McTables& T = pl->T;
+  McParams& p = pl->prm;
+  p.media = pl->media.as<Medium>() + (size_t)iL * 256;
+  p.cdfIdx = pl->cdfIdx.as<uint8_t>() + (size_t)iL * 256;
+  if (T.fresnelPossible) {
+    const int tpb = 256;
+    const int64_t nb = (T.L + tpb - 1) / tpb;
+    hipLaunchKernelGGL(mc_fill_normal_ri, dim3((unsigned)nb), dim3(tpb), 0, pl->stream, pl->normals.as<float4>(), pl->M.as<uint8_t>(),
+                       p.media, T.L);
+    MCM_HIP(hipGetLastError());
+  }
+  if (pl->sourceDistribution) {
+    // cumulative emission table in FP64 (MCmatlab.c:626-629)
+    std::vector<double>& S = pl->Shost;
+    S.assign((size_t)T.L + 1, 0.0);
+    const float* pdf = pl->sourceDistribution + (size_t)iL * T.L;
+    for (int64_t k = 1; k <= T.L; ++k) S[k] = S[k - 1] + (double)pdf[k - 1];
+    const double total = S[T.L];
+    if (!(total > 0)) return fail(MCM_ERR_INVALID_ARGUMENT, "sourceDistribution of wavelength %lld sums to zero", (long long)iL);
+    pl->power = total * T.V;
+    for (int64_t k = 1; k <= T.L; ++k) S[k] /= total;
+    S[T.L] = 1.0;
+    MCM_HIP(hipMemcpyAsync(pl->S.p, S.data(), S.size() * sizeof(double), hipMemcpyHostToDevice, pl->stream));
+  } else {
+    pl->power = T.spectrum[(size_t)iL];
+  }
+  pl->curL = (int)iL;
+  return MCM_OK;
+}
+
+int launch_transport(mcm_mc_plan* pl, uint64_t first, uint64_t n, uint64_t seed, uint32_t stream, hipStream_t s) {
+  McLaunch lp;
+  lp.firstPacket = first;
+  lp.nPackets = n;
+  lp.key0 = (uint32_t)seed;
+  lp.key1 = (uint32_t)(seed >> 32);
+  lp.stream = stream;
+  lp.moveFrac = pl->moveFrac;
+  lp.ctr = pl->counters.as<McCounters>();
+  const bool fr = pl->T.fresnelPossible;
+  const bool cr = pl->prm.criteriaActive != 0;
+  const dim3 g(pl->blocks), b(pl->threads);
+  if (fr && cr) hipLaunchKernelGGL((mc_transport<true, true>), g, b, 0, s, pl->prm, lp);
+  else if (fr) hipLaunchKernelGGL((mc_transport<true, false>), g, b, 0, s, pl->prm, lp);
+  else if (cr) hipLaunchKernelGGL((mc_transport<false, true>), g, b, 0, s, pl->prm, lp);
+  else hipLaunchKernelGGL((mc_transport<false, false>), g, b, 0, s, pl->prm, lp);
+  MCM_HIP(hipGetLastError());
+  return MCM_OK;
+}
+
+int occupancy_blocks(int device, bool fr, bool cr) {
+  int per = 0;
+  hipError_t e;
+  if (fr && cr) e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&per, (const void*)mc_transport<true, true>, 256, 0);
+  else if (fr) e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&per, (const void*)mc_transport<true, false>, 256, 0);
+  else if (cr) e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&per, (const void*)mc_transport<false, true>, 256, 0);
+  else e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&per, (const void*)mc_transport<false, false>, 256, 0);
+  if (e != hipSuccess || per < 1) per = 4;
+  return per * cu_count(device);
+}
+
+}  // namespace
+
+extern "C" {
+
+int mcm_mc_output_sizes(const mcm_mc_input* in, mcm_mc_sizes* sz) {
+  if (!in || !sz) return fail(MCM_ERR_INVALID_ARGUMENT, "null argument");
+  McTables T;
+  const std::string err = build_mc_tables(in, T);
+  if (!err.empty()) return fail(MCM_ERR_INVALID_ARGUMENT, "%s", err.c_str());
+  std::memset(sz, 0, sizeof *sz);
+  sz->NFR = T.calcNFR ? T.L * T.nL : 0;
+  sz->image = T.base.useLC ? T.nImage * T.nL : 0;
+  sz->farField = T.nFF * T.nL;
+  int64_t* ni[6] = {&sz->NI_xpos, &sz->NI_xneg, &sz->NI_ypos, &sz->NI_yneg, &sz->NI_zpos, &sz->NI_zneg};
+  for (int f = 0; f < 6; ++f) *ni[f] = T.nNI[f] * T.nL;
+  sz->image_dims[0] = T.lcRes0;
+  sz->image_dims[1] = T.lcRes0;
+  sz->image_dims[2] = T.lcRes1;
+  sz->image_dims[3] = T.nL;
+  const int64_t fd[6][2] = {{T.ny, T.nz}, {T.ny, T.nz}, {T.nx, T.nz}, {T.nx, T.nz}, {T.nx, T.ny}, {T.nx, T.ny}};
+  for (int f = 0; f < 6; ++f) {
+    sz->NI_dims[f][0] = T.nNI[f] ? (T.base.boundaryType == 2 ? kKillRange * T.nx : fd[f][0]) : 0;
+    sz->NI_dims[f][1] = T.nNI[f] ? (T.base.boundaryType == 2 ? kKillRange * T.ny : fd[f][1]) : 0;
+    sz->NI_dims[f][2] = T.nNI[f] ? T.nL : 0;
+  }
+  return MCM_OK;
+}
+
+int mcm_mc_plan_create(const mcm_mc_input* in, mcm_mc_plan** out) {
+  if (!in || !out) return fail(MCM_ERR_INVALID_ARGUMENT, "null argument");
+  *out = nullptr;
+  std::unique_ptr<mcm_mc_plan> pl(new mcm_mc_plan());
+  const std::string err = build_mc_tables(in, pl->T);
+  if (!err.empty()) return fail(MCM_ERR_INVALID_ARGUMENT, "%s", err.c_str());
+  int rc = use_device(in->GPUdevice);
+  if (rc) return rc;
+  pl->device = in->GPUdevice;
+  MCM_HIP(hipStreamCreate(&pl->stream));
+  McTables& T = pl->T;
+  hipStream_t s = pl->stream;
+  if ((rc = upload(pl->M, T.M0.data(), T.M0.size(), s))) return rc;
+  if (T.fresnelPossible && (rc = upload(pl->normals, T.normals.data(), T.normals.size(), s))) return rc;
+  if ((rc = upload(pl->media, T.media.data(), T.media.size(), s))) return rc;
+  if ((rc = upload(pl->cdfIdx, T.cdfIdx.data(), T.cdfIdx.size(), s))) return rc;
+  if ((rc = upload(pl->cdfs, T.cdfs.data(), T.cdfs.size(), s))) return rc;
+  if ((rc = upload(pl->fpid1, T.fpid1.data(), T.fpid1.size(), s))) return rc;
+  if ((rc = upload(pl->aid1, T.aid1.data(), T.aid1.size(), s))) return rc;
+  if ((rc = upload(pl->fpid2, T.fpid2.data(), T.fpid2.size(), s))) return rc;
+  if ((rc = upload(pl->aid2, T.aid2.data(), T.aid2.size(), s))) return rc;
+  if ((rc = upload(pl->muaNorm, T.muaNorm.data(), T.muaNorm.size(), s))) return rc;
+  pl->sourceDistribution = in->sourceDistribution;
+  if (in->sourceDistribution && (rc = upload<double>(pl->S, nullptr, (size_t)T.L + 1, s))) return rc;
+  if ((rc = zeros(pl->NFR, (T.calcNFR ? (size_t)T.L : 1) * sizeof(double), s))) return rc;
+  if ((rc = zeros(pl->image, (size_t)std::max<int64_t>(T.nImage, 1) * sizeof(double), s))) return rc;
+  if ((rc = zeros(pl->FF, (size_t)std::max<int64_t>(T.nFF, 1) * sizeof(double), s))) return rc;
+  for (int f = 0; f < 6; ++f)
+    if ((rc = zeros(pl->NI[f], (size_t)std::max<int64_t>(T.nNI[f], 1) * sizeof(double), s))) return rc;
+  if ((rc = zeros(pl->counters, sizeof(McCounters), s))) return rc;
+  McParams& p = pl->prm;
+  p = T.base;
+  p.M = pl->M.as<uint8_t>();
+  p.normalRI = T.fresnelPossible ? pl->normals.as<float4>() : nullptr;
+  p.cdfs = pl->cdfs.as<float>();
+  p.FPIDdist1 = pl->fpid1.as<float>();
+  p.AIDdist1 = pl->aid1.as<float>();
+  p.FPIDdist2 = pl->fpid2.as<float>();
+  p.AIDdist2 = pl->aid2.as<float>();
+  p.S = in->sourceDistribution ? pl->S.as<double>() : nullptr;
+  p.NFR = pl->NFR.as<double>();
+  p.image = pl->image.as<double>();
+  p.FF = pl->FF.as<double>();
+  for (int f = 0; f < 6; ++f) p.NI[f] = pl->NI[f].as<double>();
+  pl->threads = 256;
+  pl->blocks = occupancy_blocks(pl->device, T.fresnelPossible, p.criteriaActive != 0);
+  if (const char* e = std::getenv("MCM_MC_MOVE_FRAC")) pl->moveFrac = (float)std::atof(e);
+  if (const char* e = std::getenv("MCM_MC_BLOCKS")) pl->blocks = std::max(1, std::atoi(e));
+  if ((rc = select_wavelength(pl.get(), 0))) return rc;
+  MCM_HIP(hipStreamSynchronize(s));
+  *out = pl.release();
+  return MCM_OK;
+}
+
+int mcm_mc_plan_simulate(mcm_mc_plan* pl, int64_t iL, uint64_t nPhotons, uint64_t seed, uint64_t firstPhoton, void* stream) {
+  if (!pl) return fail(MCM_ERR_INVALID_ARGUMENT, "null plan");
+  if (iL < 0 || iL >= pl->T.nL) return fail(MCM_ERR_INVALID_ARGUMENT, "wavelength index out of range");
+  MCM_HIP(hipSetDevice(pl->device));
+  hipStream_t s = stream ? (hipStream_t)stream : pl->stream;
+  if (s != pl->stream) MCM_HIP(hipStreamSynchronize(pl->stream));
+  int rc = select_wavelength(pl, iL);
+  if (rc) return rc;
+  if (s != pl->stream) MCM_HIP(hipStreamSynchronize(pl->stream));
+  // reset the per-launch packet counter; launched/collected accumulate
+  MCM_HIP(hipMemsetAsync(pl->counters.p, 0, sizeof(unsigned long long), s));
+  if (nPhotons == 0) return MCM_OK;
+  return launch_transport(pl, firstPhoton, nPhotons, seed, (uint32_t)iL, s);
+}
+
+int mcm_mc_plan_tallies(mcm_mc_plan* pl, const double** d_deposit, uint64_t* nPhotons) {
+  if (!pl) return fail(MCM_ERR_INVALID_ARGUMENT, "null plan");
+  MCM_HIP(hipSetDevice(pl->device));
+  MCM_HIP(hipStreamSynchronize(pl->stream));
+  McCounters c;
+  MCM_HIP(hipMemcpy(&c, pl->counters.p, sizeof c, hipMemcpyDeviceToHost));
+  if (d_deposit) *d_deposit = pl->NFR.as<double>();
+  if (nPhotons) *nPhotons = c.launched;
+  return MCM_OK;
+}
+
+int mcm_mc_plan_normalize(mcm_mc_plan* pl, int64_t iL, mcm_mc_output* out) {
+  if (!pl || !out) return fail(MCM_ERR_INVALID_ARGUMENT, "null argument");
+  MCM_HIP(hipSetDevice(pl->device));
+  McTables& T = pl->T;
+  hipStream_t s = pl->stream;
+  MCM_HIP(hipDeviceSynchronize());
+  McCounters c;
+  MCM_HIP(hipMemcpy(&c, pl->counters.p, sizeof c, hipMemcpyDeviceToHost));
+  pl->stats = c;
+  // normfactor (MCmatlablib.c:1195-1200)
+  double normfactor = (double)c.launched / pl->power;
+  if (pl->sourceDistribution) normfactor /= pl->power;
+  else if (T.base.beamType == 2 && (T.base.boundaryType == 0 || T.base.boundaryType == 2)) normfactor /= kKillRange * kKillRange;
+  if (T.calcNFR && out->NFR) {
+    if (!pl->nfrOut.p) MCM_HIP(hipMalloc(&pl->nfrOut.p, (size_t)T.L * sizeof(float)));
+    const int tpb = 256;
+    const int64_t nb = (T.L + tpb - 1) / tpb;
+    hipLaunchKernelGGL(mc_normalize_nfr, dim3((unsigned)nb), dim3(tpb), 0, s, pl->NFR.as<double>(), pl->M.as<uint8_t>(),
+                       pl->muaNorm.as<double>() + (size_t)iL * T.nM, T.V * normfactor, pl->nfrOut.as<float>(), T.L);
+    MCM_HIP(hipGetLastError());
+    MCM_HIP(hipMemcpyAsync(out->NFR + (size_t)iL * T.L, pl->nfrOut.p, (size_t)T.L * sizeof(float), hipMemcpyDeviceToHost, s));
+  }
+  std::vector<double> h;
+  auto small = [&](DevBuf& b, int64_t n, float* dst, double div) -> int {
+    if (n <= 0) return MCM_OK;
+    h.resize((size_t)n);
+    MCM_HIP(hipMemcpyAsync(h.data(), b.p, (size_t)n * sizeof(double), hipMemcpyDeviceToHost, s));
+    MCM_HIP(hipStreamSynchronize(s));
+    if (dst)
+      for (int64_t k = 0; k < n; ++k) dst[k] = (float)(h[k] / div);
+    MCM_HIP(hipMemsetAsync(b.p, 0, (size_t)n * sizeof(double), s));
+    return MCM_OK;
+  };
+  int rc;
+  if (T.base.useLC) {
+    const int64_t LLC = (int64_t)T.lcRes0 * T.lcRes0;
+    const double div = LLC > 1 ? (double)T.base.lcFSorNA * (double)T.base.lcFSorNA / LLC * normfactor : normfactor;
+    if ((rc = small(pl->image, T.nImage, out->image ? out->image + (size_t)iL * T.nImage : nullptr, div))) return rc;
+  }
+  if (T.nFF && (rc = small(pl->FF, T.nFF, out->farField ? out->farField + (size_t)iL * T.nFF : nullptr, normfactor))) return rc;
+  float* ni[6] = {out->NI_xpos, out->NI_xneg, out->NI_ypos, out->NI_yneg, out->NI_zpos, out->NI_zneg};
+  const double area[6] = {T.dA[0], T.dA[0], T.dA[1], T.dA[1], T.dA[2], T.dA[2]};
+  for (int f = 0; f < 6; ++f)
+    if (T.nNI[f] && (rc = small(pl->NI[f], T.nNI[f], ni[f] ? ni[f] + (size_t)iL * T.nNI[f] : nullptr, area[f] * normfactor))) return rc;
+  MCM_HIP(hipMemsetAsync(pl->counters.p, 0, sizeof(McCounters), s));
+  MCM_HIP(hipStreamSynchronize(s));
+  pl->lastLaunched = c.launched;
+  pl->lastCollected = c.collected;
+  return MCM_OK;
+}
+
+int mcm_mc_plan_destroy(mcm_mc_plan* pl) {
+  if (!pl) return MCM_OK;
+  (void)hipSetDevice(pl->device);
+  (void)hipStreamSynchronize(pl->stream);
+  delete pl;
+  return MCM_OK;
+}
+
+// The whole MEX call: every wavelength, host buffers in and out.
+int mcm_mc_run(const mcm_mc_input* in, mcm_mc_output* out) {
+  if (!in || !out) return fail(MCM_ERR_INVALID_ARGUMENT, "null argument");
+  mcm_mc_plan* raw = nullptr;
+  int rc = mcm_mc_plan_create(in, &raw);
+  if (rc) return rc;
+  std::unique_ptr<mcm_mc_plan, int (*)(mcm_mc_plan*)> pl(raw, mcm_mc_plan_destroy);
+  McTables& T = pl->T;
+  const unsigned long long nReq = T.timed ? ~0ull : (unsigned long long)T.nPhotonsRequested;
+  double cumPhotons = 0, cumCollected = 0, cumMinutes = 0;
+  bool aborted = false, interrupted = false;
+  const int nL = T.nL;
+  for (int iL = 0; iL < nL && !aborted && !interrupted; ++iL) {
+    unsigned long long reqL = 0;
+    if (!T.timed)
+      reqL = iL == nL - 1 ? nReq - (unsigned long long)(T.requestCollected ? cumCollected : cumPhotons) : nReq / (unsigned long long)nL;
+    const double budgetUs = T.timed ? T.simulationTimeRequested * 60e6 / nL : INFINITY;
+    if ((rc = select_wavelength(pl.get(), iL))) return rc;
+    MCM_HIP(hipStreamSynchronize(pl->stream));
+    MCM_HIP(hipMemsetAsync(pl->counters.p, 0, sizeof(McCounters), pl->stream));
+    const auto t0 = std::chrono::steady_clock::now();
+    uint64_t next = 0;
+    McCounters c{};
+    // chunk size: large enough to amortise the launch tail, small enough to
+    // poll for interrupts and time budgets
+    uint64_t chunk = (uint64_t)pl->blocks * pl->threads * 32;
+    for (;;) {
+      uint64_t n;
+      if (T.timed) n = chunk;
+      else if (T.requestCollected) {
+        if (c.collected >= reqL) break;
+        const double rate = c.launched ? (double)c.collected / (double)c.launched : 0.0;
+        n = rate > 0 ? (uint64_t)std::min(4.0 * chunk, std::max(1.0, 1.1 * (reqL - c.collected) / rate)) : chunk;
+      } else {
+        if (c.launched >= reqL) break;
+        n = std::min<uint64_t>(reqL - c.launched, 8 * chunk);
+      }
+      if ((rc = mcm_mc_plan_simulate(pl.get(), iL, n, in->seed, next, nullptr))) return rc;
+      next += n;
+      MCM_HIP(hipStreamSynchronize(pl->stream));
+      MCM_HIP(hipMemcpy(&c, pl->counters.p, sizeof c, hipMemcpyDeviceToHost));
+      if (c.aborted) {
+        aborted = true;
+        break;
+      }
+      if (in->isInterruptPending && in->isInterruptPending()) {
+        interrupted = true;
+        break;
+      }
+      const double us = std::chrono::duration<double, std::micro>(std::chrono::steady_clock::now() - t0).count();
+      if (T.timed && us >= budgetUs) break;
+      if (T.timed && us > 0) chunk = std::max<uint64_t>(1024, (uint64_t)(chunk * std::min(4.0, std::max(0.25, 1e5 / us * (double)next / chunk / 8))));
+    }
+    const double minutes = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count() / 60.0;
+    cumPhotons += (double)c.launched;
+    cumCollected += (double)c.collected;
+    cumMinutes += minutes;
+    if ((rc = mcm_mc_plan_normalize(pl.get(), iL, out))) return rc;
+  }
+  out->nPhotons = cumPhotons;
+  out->nPhotonsCollected = cumCollected;
+  out->nThreads = (double)pl->blocks * pl->threads;
+  out->simulationTime = cumMinutes;
+  out->examplePaths = nullptr;
+  out->examplePathsElems = 0;
+  if (T.nExamplePaths > 0) {
+    // paths of the first packets of the first wavelength
+    if ((rc = select_wavelength(pl.get(), 0))) return rc;
+    McLaunch lp{};
+    lp.firstPacket = 0;
+    lp.nPackets = T.timed ? ~0ull : std::max<uint64_t>(1, nReq / nL);
+    lp.key0 = (uint32_t)in->seed;
+    lp.key1 = (uint32_t)(in->seed >> 32);
+    lp.stream = 0;
+    int64_t cap = 2000 * T.nExamplePaths, need = 0;
+    for (int attempt = 0; attempt < 8; ++attempt) {
+      DevBuf buf, cnt;
+      MCM_HIP(hipMalloc(&buf.p, (size_t)cap * 4 * sizeof(double)));
+      MCM_HIP(hipMalloc(&cnt.p, sizeof(int64_t)));
+      hipLaunchKernelGGL(mc_example_paths, dim3(1), dim3(64), 0, pl->stream, pl->prm, lp, (int64_t)T.nExamplePaths, buf.as<double>(),
+                         cap, cnt.as<int64_t>());
+      MCM_HIP(hipGetLastError());
+      MCM_HIP(hipMemcpyAsync(&need, cnt.p, sizeof need, hipMemcpyDeviceToHost, pl->stream));
+      MCM_HIP(hipStreamSynchronize(pl->stream));
+      if (need <= cap) {
+        double* host = (double*)std::malloc((size_t)std::max<int64_t>(need, 1) * 4 * sizeof(double));
+        if (!host) return fail(MCM_ERR_OUT_OF_MEMORY, "Error: Out of memory");
+        MCM_HIP(hipMemcpy(host, buf.p, (size_t)need * 4 * sizeof(double), hipMemcpyDeviceToHost));
+        out->examplePaths = host;
+        out->examplePathsElems = need;
+        break;
+      }
+      cap = need + 16;
+    }
+  }
+  if (aborted) set_error("all packets launch outside the simulation cuboid");
+  return MCM_OK;
+}
+
+}  // extern "C"
