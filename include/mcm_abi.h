@@ -192,6 +192,12 @@ int mcm_mc_plan_normalize(mcm_mc_plan *plan, int64_t iL, mcm_mc_output *out);
 /* Raw FP64 tallies on the device (absorbed weight per voxel, before
  * normalisation), valid until the next plan call. */
 int mcm_mc_plan_tallies(mcm_mc_plan *plan, const double **d_deposit, uint64_t *nPhotons);
+/* Event counters accumulated since the last plan_normalize (synchronises):
+ * [0] packets launched alive, [1] collected, [2] voxel sub-steps,
+ * [3] scatterings, [4] interface (Fresnel) events, [5] packets retired at the
+ * sub-step cap, [6] aborted (1 = a packet failed 10^6 launch attempts),
+ * [7] lanes per launch. */
+int mcm_mc_plan_counters(mcm_mc_plan *plan, double counters[8]);
 int mcm_mc_plan_destroy(mcm_mc_plan *plan);
 
 /* ---- heat ---- */

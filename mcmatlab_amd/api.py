@@ -112,6 +112,13 @@ class MCPlan:
         check(lib().mcm_mc_plan_tallies(self.h, C.byref(p), C.byref(n)))
         return p.value, n.value
 
+    def counters(self):
+        """Event counters since the last normalize (mcm_mc_plan_counters)."""
+        buf = (C.c_double * 8)()
+        check(lib().mcm_mc_plan_counters(self.h, buf))
+        keys = ("launched", "collected", "substeps", "scatterings", "interfaces", "runaway", "aborted", "lanes")
+        return dict(zip(keys, buf[:]))
+
     def close(self):
         if self.h:
             lib().mcm_mc_plan_destroy(self.h)

@@ -106,136 +106,126 @@ __device__ inline void fold_lds_max(unsigned long long* s, unsigned long long* g
     if (s[k] != none) atomicMax(&g[k], s[k]);
 }
 
-// substeps 1a + 1b (finiteElementHeatPropagator.c:288-312, :352-389): lane per (y, z) x-line
+enum : int { SWEEP_X = 0, SWEEP_Y = 1, SWEEP_Z = 2 };
+constexpr int kLdsCoef = 1536;  // coefficient tables up to 22 media live in LDS
+
 template <typename RT>
-__global__ void __launch_bounds__(kHeatTPB) heat_sweep_x(HeatArgs<RT> a) {
-  const int line = blockIdx.x * blockDim.x + threadIdx.x;
-  if (line >= a.ny * a.nz) return;
-  const int y = line % a.ny, z = line / a.ny;
-  const int nx = a.nx, nxy = a.nx * a.ny;
-  const int base = y * nx + z * nxy;
-  const RT dt = a.dt;
-  RT tprev = 0, bprev = 1;
-  for (int x = 0; x < nx; ++x) {
-    const int i = base + x;
-    const int m = a.M[i];
-    const bool fz = frozen(a, x, y, z);
-    RT dTdt = 0;
-    if (!fz) {
-      dTdt = a.lightsOn ? (RT)a.Q[i] : (RT)0;
-      if (x != 0) dTdt += (a.T1[i - 1] - a.T1[i]) * cc(a, m, a.M[i - 1], 0) / 2;
-      if (x != nx - 1) dTdt += (a.T1[i + 1] - a.T1[i]) * cc(a, m, a.M[i + 1], 0) / 2;
-      if (y != 0) dTdt += (a.T1[i - nx] - a.T1[i]) * cc(a, m, a.M[i - nx], 1);
-      if (y != a.ny - 1) dTdt += (a.T1[i + nx] - a.T1[i]) * cc(a, m, a.M[i + nx], 1);
-      if (z != 0) dTdt += (a.T1[i - nxy] - a.T1[i]) * cc(a, m, a.M[i - nxy], 2);
-      if (z != a.nz - 1) dTdt += (a.T1[i + nxy] - a.T1[i]) * cc(a, m, a.M[i + nxy], 2);
-    }
-    RT t = a.T1[i] + dt * dTdt;
-    RT b = 1;
-    if (!fz) {
-      if (x < nx - 1) b += dt / 2 * cc(a, m, a.M[i + 1], 0);
-      if (x > 0) {
-        const int mp = a.M[i - 1];
-        b += dt / 2 * cc(a, m, mp, 0);
-        const RT w = -dt / 2 * cc(a, m, mp, 0) / bprev;
-        if (x > 1 || !a.sinked) b += w * dt / 2 * cc(a, mp, m, 0);
-        t -= w * tprev;
-      }
-    }
-    a.T2[i] = t;
-    a.B[i] = b;
-    tprev = t;
-    bprev = b;
-  }
-  RT xnext = 0;
-  for (int x = nx - 1; x >= 0; --x) {
-    const int i = base + x;
-    RT t = a.T2[i];
-    if (!frozen(a, x, y, z)) {
-      t = (t + (x == nx - 1 ? (RT)0 : dt / 2 * cc(a, a.M[i], a.M[i + 1], 0) * xnext)) / a.B[i];
-      a.T2[i] = t;
-    }
-    xnext = t;
-  }
+__device__ inline RT coef(const RT* c, int nM, int m, int nb, int d) {
+  return c[m + nb * nM + d * nM * nM];
 }
 
-// substeps 2a + 2b (:429-448, :486-521): lane per (x, z) y-line
+// Per-workgroup copies of the small per-medium tables.
 template <typename RT>
-__global__ void __launch_bounds__(kHeatTPB) heat_sweep_y(HeatArgs<RT> a) {
-  const int line = blockIdx.x * blockDim.x + threadIdx.x;
-  if (line >= a.nx * a.nz) return;
-  const int x = line % a.nx, z = line / a.nx;
-  const int nx = a.nx, ny = a.ny, nxy = a.nx * a.ny;
-  const int base = x + z * nxy;
-  const RT dt = a.dt;
-  RT tprev = 0, bprev = 1;
-  for (int y = 0; y < ny; ++y) {
-    const int i = base + y * nx;
-    const int m = a.M[i];
-    const bool fz = frozen(a, x, y, z);
-    RT t = a.T2[i];
-    RT b = 1;
-    if (!fz) {
-      RT dTdt = 0;
-      if (y != 0) dTdt -= (a.T1[i - nx] - a.T1[i]) * cc(a, m, a.M[i - nx], 1) / 2;
-      if (y != ny - 1) dTdt -= (a.T1[i + nx] - a.T1[i]) * cc(a, m, a.M[i + nx], 1) / 2;
-      t = t + dt * dTdt;
-      if (y < ny - 1) b += dt / 2 * cc(a, m, a.M[i + nx], 1);
-      if (y > 0) {
-        const int mp = a.M[i - nx];
-        b += dt / 2 * cc(a, m, mp, 1);
-        const RT w = -dt / 2 * cc(a, m, mp, 1) / bprev;
-        if (y > 1 || !a.sinked) b += w * dt / 2 * cc(a, mp, m, 1);
-        t -= w * tprev;
-      }
-    }
-    a.T2[i] = t;
-    a.B[i] = b;
-    tprev = t;
-    bprev = b;
+struct HeatLds {
+  RT c[kLdsCoef];
+  RT logA[256];
+  RT E[256];
+  unsigned long long maxKey[256];
+};
+
+template <typename RT>
+__device__ inline const RT* stage_tables(const HeatArgs<RT>& a, HeatLds<RT>& s) {
+  const int nc = a.nM * a.nM * 3;
+  if (nc <= kLdsCoef)
+    for (int k = threadIdx.x; k < nc; k += blockDim.x) s.c[k] = a.c[k];
+  for (int k = threadIdx.x; k < a.nM; k += blockDim.x) {
+    s.logA[k] = a.logA[k];
+    s.E[k] = a.E[k];
+    s.maxKey[k] = order_key(-INFINITY);
   }
-  RT xnext = 0;
-  for (int y = ny - 1; y >= 0; --y) {
-    const int i = base + y * nx;
-    RT t = a.T2[i];
-    if (!frozen(a, x, y, z)) {
-      t = (t + (y == ny - 1 ? (RT)0 : dt / 2 * cc(a, a.M[i], a.M[i + nx], 1) * xnext)) / a.B[i];
-      a.T2[i] = t;
-    }
-    xnext = t;
-  }
+  __syncthreads();
+  return nc <= kLdsCoef ? s.c : a.c;
 }
 
-// substeps 3a + 3b (:546-565, :606-646) with the Arrhenius update and the
-// per-medium maximum: lane per (x, y) z-line
-template <typename RT>
-__global__ void __launch_bounds__(kHeatTPB) heat_sweep_z(HeatArgs<RT> a) {
-  __shared__ unsigned long long sMax[256];
-  init_lds_max(sMax, a.nM);
-  const int line = blockIdx.x * blockDim.x + threadIdx.x;
-  if (line < a.nx * a.ny) {
-    const int x = line % a.nx, y = line / a.nx;
-    const int nz = a.nz, nxy = a.nx * a.ny;
-    const int base = line;
-    const RT dt = a.dt;
-    RT tprev = 0, bprev = 1;
-    for (int z = 0; z < nz; ++z) {
-      const int i = base + z * nxy;
-      const int m = a.M[i];
-      const bool fz = frozen(a, x, y, z);
-      RT t = a.T2[i];
+// One grid line: the explicit part of the substep (KIND X: the full
+// right-hand side of substep 1a; Y/Z: the -dt/2 L_d T correction of 2a/3a),
+// Thomas forward elimination, back substitution, and for Z the Arrhenius
+// update and the running maxima.  Expressions and their order are those of
+// the reference (finiteElementHeatPropagator.c:288-646); only the memory
+// access is reorganised: each lane loads a block of U consecutive elements
+// of its line (and their stencil neighbours) before running the sequential
+// recurrence over them, so a wavefront has U loads in flight instead of one.
+//   i0, st, n : first voxel, stride and length of the line (axis d)
+//   lf        : the whole line lies on a fixed-temperature boundary
+//   s1/k1/n1, s2/k2/n2 : stride, coordinate and extent of the two other axes
+template <typename RT, int KIND, int U>
+__device__ void sweep_line(const HeatArgs<RT>& a, const RT* cT, const HeatLds<RT>& L, int i0, int st, int n, bool lf, int d, int s1,
+                           int k1, int n1, int s2, int k2, int n2, MaxRun* run, unsigned long long* sMax) {
+  const int nM = a.nM;
+  const RT dt = a.dt;
+  const bool sk = a.sinked;
+  RT tprev = 0, bprev = 1;
+  for (int k0 = 0; k0 < n; k0 += U) {
+    RT Tw[U + 2];
+    int Mw[U + 2];
+    RT R0[U], Ta[U], Tb[U], Tc[U], Td[U];
+    int Ma[U], Mb[U], Mc[U], Md[U];
+    float Qv[U];
+#pragma unroll
+    for (int u = 0; u < U + 2; ++u) {
+      const int k = k0 - 1 + u;
+      const bool ok = k >= 0 && k < n;
+      const int i = i0 + k * st;
+      Tw[u] = ok ? a.T1[i] : (RT)0;
+      Mw[u] = ok ? (int)a.M[i] : 0;
+    }
+#pragma unroll
+    for (int u = 0; u < U; ++u) {
+      const int k = k0 + u;
+      const bool ok = k < n;
+      const int i = i0 + k * st;
+      if (KIND == SWEEP_X) {
+        const bool a1 = ok && k1 > 0, b1 = ok && k1 < n1 - 1, a2 = ok && k2 > 0, b2 = ok && k2 < n2 - 1;
+        Ta[u] = a1 ? a.T1[i - s1] : (RT)0;
+        Ma[u] = a1 ? (int)a.M[i - s1] : 0;
+        Tb[u] = b1 ? a.T1[i + s1] : (RT)0;
+        Mb[u] = b1 ? (int)a.M[i + s1] : 0;
+        Tc[u] = a2 ? a.T1[i - s2] : (RT)0;
+        Mc[u] = a2 ? (int)a.M[i - s2] : 0;
+        Td[u] = b2 ? a.T1[i + s2] : (RT)0;
+        Md[u] = b2 ? (int)a.M[i + s2] : 0;
+        Qv[u] = ok ? a.Q[i] : 0.f;
+      } else {
+        R0[u] = ok ? a.T2[i] : (RT)0;
+      }
+    }
+#pragma unroll
+    for (int u = 0; u < U; ++u) {
+      const int k = k0 + u;
+      if (k >= n) break;
+      const int i = i0 + k * st;
+      const int m = Mw[u + 1];
+      const bool fz = lf || (sk && (k == 0 || k == n - 1));
+      RT t;
+      if (KIND == SWEEP_X) {
+        RT dTdt = 0;
+        if (!fz) {
+          dTdt = a.lightsOn ? (RT)Qv[u] : (RT)0;
+          if (k != 0) dTdt += (Tw[u] - Tw[u + 1]) * coef(cT, nM, m, Mw[u], 0) / 2;
+          if (k != n - 1) dTdt += (Tw[u + 2] - Tw[u + 1]) * coef(cT, nM, m, Mw[u + 2], 0) / 2;
+          if (k1 != 0) dTdt += (Ta[u] - Tw[u + 1]) * coef(cT, nM, m, Ma[u], 1);
+          if (k1 != n1 - 1) dTdt += (Tb[u] - Tw[u + 1]) * coef(cT, nM, m, Mb[u], 1);
+          if (k2 != 0) dTdt += (Tc[u] - Tw[u + 1]) * coef(cT, nM, m, Mc[u], 2);
+          if (k2 != n2 - 1) dTdt += (Td[u] - Tw[u + 1]) * coef(cT, nM, m, Md[u], 2);
+        }
+        t = Tw[u + 1] + dt * dTdt;
+      } else {
+        t = R0[u];
+        if (!fz) {
+          RT dTdt = 0;
+          if (k != 0) dTdt -= (Tw[u] - Tw[u + 1]) * coef(cT, nM, m, Mw[u], d) / 2;
+          if (k != n - 1) dTdt -= (Tw[u + 2] - Tw[u + 1]) * coef(cT, nM, m, Mw[u + 2], d) / 2;
+          t = t + dt * dTdt;
+        }
+      }
       RT b = 1;
       if (!fz) {
-        RT dTdt = 0;
-        if (z != 0) dTdt -= (a.T1[i - nxy] - a.T1[i]) * cc(a, m, a.M[i - nxy], 2) / 2;
-        if (z != nz - 1) dTdt -= (a.T1[i + nxy] - a.T1[i]) * cc(a, m, a.M[i + nxy], 2) / 2;
-        t = t + dt * dTdt;
-        if (z < nz - 1) b += dt / 2 * cc(a, m, a.M[i + nxy], 2);
-        if (z > 0) {
-          const int mp = a.M[i - nxy];
-          b += dt / 2 * cc(a, m, mp, 2);
-          const RT w = -dt / 2 * cc(a, m, mp, 2) / bprev;
-          if (z > 1 || !a.sinked) b += w * dt / 2 * cc(a, mp, m, 2);
+        if (k < n - 1) b += dt / 2 * coef(cT, nM, m, Mw[u + 2], d);
+        if (k > 0) {
+          const int mp = Mw[u];
+          b += dt / 2 * coef(cT, nM, m, mp, d);
+          const RT w = -dt / 2 * coef(cT, nM, m, mp, d) / bprev;
+          if (k > 1 || !sk) b += w * dt / 2 * coef(cT, nM, mp, m, d);
           t -= w * tprev;
         }
       }
@@ -244,67 +234,139 @@ __global__ void __launch_bounds__(kHeatTPB) heat_sweep_z(HeatArgs<RT> a) {
       tprev = t;
       bprev = b;
     }
-    RT xnext = 0;
-    MaxRun run;
-    for (int z = nz - 1; z >= 0; --z) {
-      const int i = base + z * nxy;
-      const int m = a.M[i];
-      RT t = a.T2[i];
-      if (!frozen(a, x, y, z)) {
-        t = (t + (z == nz - 1 ? (RT)0 : dt / 2 * cc(a, m, a.M[i + nxy], 2) * xnext)) / a.B[i];
+  }
+  RT xnext = 0;
+  for (int kt = n - 1; kt >= 0; kt -= U) {
+    RT Dv[U], Bv[U], T1v[U], Omv[U];
+    int Mi[U], Mn[U];
+#pragma unroll
+    for (int u = 0; u < U; ++u) {
+      const int k = kt - u;
+      const bool ok = k >= 0;
+      const int i = i0 + k * st;
+      Dv[u] = ok ? a.T2[i] : (RT)0;
+      Bv[u] = ok ? a.B[i] : (RT)1;
+      Mi[u] = ok ? (int)a.M[i] : 0;
+      Mn[u] = ok && k < n - 1 ? (int)a.M[i + st] : 0;
+      if (KIND == SWEEP_Z) {
+        T1v[u] = ok ? a.T1[i] : (RT)0;
+        Omv[u] = ok && a.calcDamage ? a.Om[i] : (RT)0;
+      }
+    }
+#pragma unroll
+    for (int u = 0; u < U; ++u) {
+      const int k = kt - u;
+      if (k < 0) break;
+      const int i = i0 + k * st;
+      const int m = Mi[u];
+      const bool fz = lf || (sk && (k == 0 || k == n - 1));
+      RT t = Dv[u];
+      if (!fz) {
+        t = (t + (k == n - 1 ? (RT)0 : dt / 2 * coef(cT, nM, m, Mn[u], d) * xnext)) / Bv[u];
         a.T2[i] = t;
       }
       xnext = t;
-      if (a.calcDamage && a.E[m] != 0) a.Om[i] += dt * (RT)exp(a.logA[m] - a.E[m] / (a.Rgas * ((t + a.T1[i]) / 2 + a.czero)));
-      run.add(m, (double)t, sMax);
+      if (KIND == SWEEP_Z) {
+        if (a.calcDamage && L.E[m] != 0) a.Om[i] = Omv[u] + dt * (RT)exp(L.logA[m] - L.E[m] / (a.Rgas * ((t + T1v[u]) / 2 + a.czero)));
+        run->add(m, (double)t, sMax);
+      }
     }
-    run.flush(sMax);
   }
-  fold_lds_max(sMax, a.maxKey, a.nM);
 }
 
-// Forward Euler step (no reference counterpart; same neighbour order and
-// damage/maximum update as the DG-ADI step): lane per (x, y) column, z
-// neighbours carried in registers.
+constexpr int kSweepU = 8;
+
+// substeps 1a + 1b (finiteElementHeatPropagator.c:288-312, :352-389): lane per (y, z) x-line
 template <typename RT>
-__global__ void __launch_bounds__(kHeatTPB) heat_explicit(HeatArgs<RT> a) {
-  __shared__ unsigned long long sMax[256];
-  init_lds_max(sMax, a.nM);
+__global__ void __launch_bounds__(kHeatTPB) heat_sweep_x(HeatArgs<RT> a) {
+  __shared__ HeatLds<RT> L;
+  const RT* cT = stage_tables(a, L);
+  const int line = blockIdx.x * blockDim.x + threadIdx.x;
+  if (line >= a.ny * a.nz) return;
+  const int y = line % a.ny, z = line / a.ny;
+  const bool lf = a.sinked && (y == 0 || y == a.ny - 1 || z == 0 || z == a.nz - 1);
+  sweep_line<RT, SWEEP_X, kSweepU>(a, cT, L, y * a.nx + z * a.nx * a.ny, 1, a.nx, lf, 0, a.nx, y, a.ny, a.nx * a.ny, z, a.nz, nullptr,
+                                   nullptr);
+}
+
+// substeps 2a + 2b (:429-448, :486-521): lane per (x, z) y-line, lanes along x
+template <typename RT>
+__global__ void __launch_bounds__(kHeatTPB) heat_sweep_y(HeatArgs<RT> a) {
+  __shared__ HeatLds<RT> L;
+  const RT* cT = stage_tables(a, L);
+  const int line = blockIdx.x * blockDim.x + threadIdx.x;
+  if (line >= a.nx * a.nz) return;
+  const int x = line % a.nx, z = line / a.nx;
+  const bool lf = a.sinked && (x == 0 || x == a.nx - 1 || z == 0 || z == a.nz - 1);
+  sweep_line<RT, SWEEP_Y, kSweepU>(a, cT, L, x + z * a.nx * a.ny, a.nx, a.ny, lf, 1, 0, 0, 1, 0, 0, 1, nullptr, nullptr);
+}
+
+// substeps 3a + 3b (:546-565, :606-646) with the Arrhenius update and the
+// per-medium maximum: lane per (x, y) z-line, lanes along x
+template <typename RT>
+__global__ void __launch_bounds__(kHeatTPB) heat_sweep_z(HeatArgs<RT> a) {
+  __shared__ HeatLds<RT> L;
+  const RT* cT = stage_tables(a, L);
   const int line = blockIdx.x * blockDim.x + threadIdx.x;
   if (line < a.nx * a.ny) {
     const int x = line % a.nx, y = line / a.nx;
-    const int nx = a.nx, ny = a.ny, nz = a.nz, nxy = nx * ny;
+    const bool lf = a.sinked && (x == 0 || x == a.nx - 1 || y == 0 || y == a.ny - 1);
+    MaxRun run;
+    sweep_line<RT, SWEEP_Z, kSweepU>(a, cT, L, line, a.nx * a.ny, a.nz, lf, 2, 0, 0, 1, 0, 0, 1, &run, L.maxKey);
+    run.flush(L.maxKey);
+  }
+  fold_lds_max(L.maxKey, a.maxKey, a.nM);
+}
+
+constexpr int kExplicitZ = 8;
+
+// Forward Euler step (no reference counterpart; same neighbour order and
+// damage/maximum update as the DG-ADI step): lane per (x, y, chunk of 8 z),
+// lanes along x, z neighbours carried in registers along the chunk.
+template <typename RT>
+__global__ void __launch_bounds__(kHeatTPB) heat_explicit(HeatArgs<RT> a) {
+  __shared__ HeatLds<RT> L;
+  const RT* cT = stage_tables(a, L);
+  const int nx = a.nx, ny = a.ny, nz = a.nz, nxy = nx * ny, nM = a.nM;
+  const int nzc = (nz + kExplicitZ - 1) / kExplicitZ;
+  const int lane = blockIdx.x * blockDim.x + threadIdx.x;
+  if (lane < nxy * nzc) {
+    const int col = lane % nxy, zc = lane / nxy;
+    const int x = col % nx, y = col / nx;
+    const int z0 = zc * kExplicitZ;
     const RT dt = a.dt;
     MaxRun run;
-    RT tBelow = 0, tHere = a.T1[line];
-    int mBelow = 0, mHere = a.M[line];
-    for (int z = 0; z < nz; ++z) {
-      const int i = line + z * nxy;
+    RT tBelow = z0 > 0 ? a.T1[col + (z0 - 1) * nxy] : (RT)0;
+    int mBelow = z0 > 0 ? (int)a.M[col + (z0 - 1) * nxy] : 0;
+    RT tHere = a.T1[col + z0 * nxy];
+    int mHere = a.M[col + z0 * nxy];
+    for (int z = z0; z < z0 + kExplicitZ && z < nz; ++z) {
+      const int i = col + z * nxy;
       const RT tAbove = z != nz - 1 ? a.T1[i + nxy] : (RT)0;
-      const int mAbove = z != nz - 1 ? a.M[i + nxy] : 0;
+      const int mAbove = z != nz - 1 ? (int)a.M[i + nxy] : 0;
       RT dTdt = 0;
       if (!frozen(a, x, y, z)) {
         dTdt = a.lightsOn ? (RT)a.Q[i] : (RT)0;
-        if (x != 0) dTdt += (a.T1[i - 1] - tHere) * cc(a, mHere, a.M[i - 1], 0);
-        if (x != nx - 1) dTdt += (a.T1[i + 1] - tHere) * cc(a, mHere, a.M[i + 1], 0);
-        if (y != 0) dTdt += (a.T1[i - nx] - tHere) * cc(a, mHere, a.M[i - nx], 1);
-        if (y != ny - 1) dTdt += (a.T1[i + nx] - tHere) * cc(a, mHere, a.M[i + nx], 1);
-        if (z != 0) dTdt += (tBelow - tHere) * cc(a, mHere, mBelow, 2);
-        if (z != nz - 1) dTdt += (tAbove - tHere) * cc(a, mHere, mAbove, 2);
+        if (x != 0) dTdt += (a.T1[i - 1] - tHere) * coef(cT, nM, mHere, a.M[i - 1], 0);
+        if (x != nx - 1) dTdt += (a.T1[i + 1] - tHere) * coef(cT, nM, mHere, a.M[i + 1], 0);
+        if (y != 0) dTdt += (a.T1[i - nx] - tHere) * coef(cT, nM, mHere, a.M[i - nx], 1);
+        if (y != ny - 1) dTdt += (a.T1[i + nx] - tHere) * coef(cT, nM, mHere, a.M[i + nx], 1);
+        if (z != 0) dTdt += (tBelow - tHere) * coef(cT, nM, mHere, mBelow, 2);
+        if (z != nz - 1) dTdt += (tAbove - tHere) * coef(cT, nM, mHere, mAbove, 2);
       }
       const RT t = tHere + dt * dTdt;
       a.T2[i] = t;
-      if (a.calcDamage && a.E[mHere] != 0)
-        a.Om[i] += dt * (RT)exp(a.logA[mHere] - a.E[mHere] / (a.Rgas * ((t + tHere) / 2 + a.czero)));
-      run.add(mHere, (double)t, sMax);
+      if (a.calcDamage && L.E[mHere] != 0)
+        a.Om[i] += dt * (RT)exp(L.logA[mHere] - L.E[mHere] / (a.Rgas * ((t + tHere) / 2 + a.czero)));
+      run.add(mHere, (double)t, L.maxKey);
       tBelow = tHere;
       mBelow = mHere;
       tHere = tAbove;
       mHere = mAbove;
     }
-    run.flush(sMax);
+    run.flush(L.maxKey);
   }
-  fold_lds_max(sMax, a.maxKey, a.nM);
+  fold_lds_max(L.maxKey, a.maxKey, a.nM);
 }
 
 // recordTempSensors (:652-663): trilinear interpolation of the new T into
@@ -445,7 +507,8 @@ int advance(mcm_heat_plan* P, int64_t steps, double dt, int lightsOn) {
     }
     HeatArgs<RT> a = make_args<RT>(P, dt, lightsOn);
     if (P->explicitScheme) {
-      hipLaunchKernelGGL(heat_explicit<RT>, dim3(gz), dim3(tpb), 0, P->stream, a);
+      const unsigned ge = (unsigned)(((int64_t)P->nx * P->ny * ((P->nz + kExplicitZ - 1) / kExplicitZ) + tpb - 1) / tpb);
+      hipLaunchKernelGGL(heat_explicit<RT>, dim3(ge), dim3(tpb), 0, P->stream, a);
     } else {
       hipLaunchKernelGGL(heat_sweep_x<RT>, dim3(gx), dim3(tpb), 0, P->stream, a);
       hipLaunchKernelGGL(heat_sweep_y<RT>, dim3(gy), dim3(tpb), 0, P->stream, a);

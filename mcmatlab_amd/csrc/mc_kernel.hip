@@ -563,6 +563,23 @@ int mcm_mc_plan_normalize(mcm_mc_plan* pl, int64_t iL, mcm_mc_output* out) {
   return MCM_OK;
 }
 
+int mcm_mc_plan_counters(mcm_mc_plan* pl, double* out) {
+  if (!pl || !out) return fail(MCM_ERR_INVALID_ARGUMENT, "null argument");
+  MCM_HIP(hipSetDevice(pl->device));
+  MCM_HIP(hipDeviceSynchronize());
+  McCounters c;
+  MCM_HIP(hipMemcpy(&c, pl->counters.p, sizeof c, hipMemcpyDeviceToHost));
+  out[0] = (double)c.launched;
+  out[1] = (double)c.collected;
+  out[2] = (double)c.substeps;
+  out[3] = (double)c.scatters;
+  out[4] = (double)c.interfaces;
+  out[5] = (double)c.runaway;
+  out[6] = (double)c.aborted;
+  out[7] = (double)pl->blocks * pl->threads;
+  return MCM_OK;
+}
+
 int mcm_mc_plan_destroy(mcm_mc_plan* pl) {
   if (!pl) return MCM_OK;
   (void)hipSetDevice(pl->device);

@@ -93,3 +93,21 @@ def compare_voxels(runs_a, runs_b, key="NFR", min_nonzero=None):
     if t.size == 0:
         return 0.0, 0.0, 0
     return float(np.mean(np.abs(t) > 3)), float(np.mean(t ** 2)), int(t.size)
+
+
+def check_scalars_two_stage(run_a, run_b, model, seeds=8, extra=24, simType=1, iL=0, runs=None):
+    """3-sigma check of every scalar summary with a second stage: a summary
+    outside the bound after `seeds` runs per side is re-tested on
+    `seeds + extra` runs per side, and only fails if it is still outside.
+    Keeps the false-alarm rate of many simultaneous 3-sigma comparisons low
+    while a real bias (which does not shrink with more runs) still fails.
+    Returns the list of failures [(name, t, t_crit, mean_a, mean_b)]."""
+    A = runs[0] if runs else [run_a(s) for s in range(seeds)]
+    B = runs[1] if runs else [run_b(s) for s in range(seeds)]
+    bad = [r for r in compare_scalars(A, B, model, simType, iL) if not abs(r[1]) <= r[2]]
+    if not bad:
+        return []
+    A = A + [run_a(s) for s in range(seeds, seeds + extra)]
+    B = B + [run_b(s) for s in range(seeds, seeds + extra)]
+    names = {r[0] for r in bad}
+    return [r for r in compare_scalars(A, B, model, simType, iL) if r[0] in names and not abs(r[1]) <= r[2]]

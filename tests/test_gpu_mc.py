@@ -17,7 +17,7 @@ import pytest
 
 import models
 import oracle_lib as OL
-from mcstats import compare_scalars, compare_voxels
+from mcstats import check_scalars_two_stage, compare_scalars, compare_voxels
 
 pytestmark = pytest.mark.gpu
 
@@ -62,7 +62,7 @@ def test_gpu_matches_oracle(api, name):
     O = [OL.oracle_mc(m, seed=s) for s in range(SEEDS)]
     for g in G:
         assert g["nPhotons"] == NPH
-    bad = [(k, t, tc, a, b) for k, t, tc, a, b in compare_scalars(G, O, m) if not abs(t) <= tc]
+    bad = check_scalars_two_stage(lambda s: gpu_runs(api, m, [s])[0], lambda s: OL.oracle_mc(m, seed=s), m, runs=(G, O))
     assert not bad, f"scalar summaries outside 3 sigma: {bad}"
     frac, mt2, n = compare_voxels(G, O)
     assert n > 100
@@ -78,8 +78,8 @@ def test_gpu_wavelengths(api):
     O = [OL.oracle_mc(m, seed=s) for s in range(SEEDS)]
     assert G[0]["nPhotons"] == 3 * NPH + 1
     for iL in range(3):
-        rows = compare_scalars(G, O, m, iL=iL)
-        bad = [r for r in rows if not abs(r[1]) <= r[2]]
+        bad = check_scalars_two_stage(lambda s: gpu_runs(api, m, [s])[0], lambda s: OL.oracle_mc(m, seed=s), m, iL=iL,
+                                      runs=(G, O))
         assert not bad, (iL, bad)
 
 

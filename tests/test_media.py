@@ -1,0 +1,45 @@
+"""Host-side input builders (mcmatlab_amd/media.py) against hand-computed
+values of the reference's MATLAB formulas."""
+import math
+
+import numpy as np
+
+from mcmatlab_amd.media import heat_coefficients, heat_dtmax, interface_normals, sensor_weights
+
+
+def test_interface_normals_planar_interface_points_into_each_medium():
+    M = np.ones((9, 9, 12), np.uint8)
+    M[:, :, 6:] = 2
+    IN = interface_normals(M, [1.0, 1.5], (0.01, 0.01, 0.01))
+    theta = IN[0].reshape(M.shape, order="F")
+    # the n=1.5 class: gradient of its indicator points +z (theta = 0);
+    # the n=1 class: -z (theta = pi)
+    assert np.allclose(theta[4, 4, 6], 0.0, atol=1e-6)
+    assert np.allclose(theta[4, 4, 5], math.pi, atol=1e-6)
+
+
+def test_heat_coefficients_harmonic_mean_over_vhc():
+    VHC, TC = [3.7, 1.2], [0.0037, 0.0011]
+    c = heat_coefficients(VHC, TC, 0.01, 0.02, 0.04)
+    eff = 2 * TC[0] * TC[1] / (TC[0] + TC[1])
+    np.testing.assert_allclose(c[0, 1, 0], eff / VHC[0] / 0.01 ** 2, rtol=1e-6)
+    np.testing.assert_allclose(c[1, 0, 2], eff / VHC[1] / 0.04 ** 2, rtol=1e-6)
+    np.testing.assert_allclose(c[0, 0, 1], TC[0] / VHC[0] / 0.02 ** 2, rtol=1e-6)
+    assert c.dtype == np.float32 and c.shape == (2, 2, 3)
+
+
+def test_heat_dtmax_homogeneous_interior():
+    M = np.ones((6, 6, 6), np.uint8)
+    d = 5e-4
+    dtmax = heat_dtmax(M, [3.7], [0.0037], d, d, d)
+    # an interior voxel has 6 neighbours: VHC / (6 TC / d^2)
+    np.testing.assert_allclose(dtmax, 3.7 / (6 * 0.0037 / d ** 2), rtol=1e-6)
+
+
+def test_sensor_weights_trilinear_corner():
+    x = np.arange(5) * 0.1
+    li, w = sensor_weights([[0.15, 0.2, 0.05], [-1, 9, 0.0]], x, x, x)
+    assert li[0] == 1 + 2 * 5 + 0 * 25
+    np.testing.assert_allclose(w[0], [0.5, 0.0, 0.5], atol=1e-12)
+    assert li[1] == 0 + 3 * 5 + 0
+    np.testing.assert_allclose(w[1], [0.0, 1.0, 0.0])
