@@ -19,6 +19,7 @@ line (see the keys below), including:
                 the heat kernels are built.
 
 Usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--packets N]
+       python bench.py --copy-probe BYTES   (plain copy kernel for counter scaling)
 """
 from __future__ import annotations
 
@@ -71,6 +72,22 @@ class Hip:
         self._ok(self.h.hipEventSynchronize(b), "hipEventSynchronize")
         self._ok(self.h.hipEventElapsedTime(C.byref(ms), a, b), "hipEventElapsedTime")
         return ms.value
+
+    def copy_probe(self, nbytes, repeats=3):
+        """Device-to-device copies of nbytes (the runtime's plain copy
+        kernel): the reference point that scales the HBM byte counters."""
+        a, b = C.c_void_p(), C.c_void_p()
+        self._ok(self.h.hipMalloc(C.byref(a), C.c_size_t(nbytes)), "hipMalloc")
+        self._ok(self.h.hipMalloc(C.byref(b), C.c_size_t(nbytes)), "hipMalloc")
+        self.h.hipMemcpy.argtypes = [C.c_void_p, C.c_void_p, C.c_size_t, C.c_int]
+        self.h.hipMemset.argtypes = [C.c_void_p, C.c_int, C.c_size_t]
+        self._ok(self.h.hipMemset(a, 1, C.c_size_t(nbytes)), "hipMemset")
+        for _ in range(repeats):
+            self._ok(self.h.hipMemcpy(b, a, C.c_size_t(nbytes), 3), "hipMemcpy")  # 3 = device to device
+        self.sync()
+        self.h.hipFree.argtypes = [C.c_void_p]
+        self.h.hipFree(a)
+        self.h.hipFree(b)
 
     def sync(self, s=None):
         if s is None:
@@ -147,10 +164,17 @@ def main():
     ap.add_argument("--packets", type=float, default=1e8)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-heat", action="store_true")
+    ap.add_argument("--copy-probe", type=int, default=0, metavar="BYTES",
+                    help="only run device-to-device copies of BYTES (counter calibration) and exit")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
         print("bench.py measures one GPU only: this build has no multi-GPU path (run with --gpus 1)", file=sys.stderr)
         sys.exit(2)
+
+    if args.copy_probe:
+        Hip().copy_probe(args.copy_probe)
+        print(json.dumps({"copy_probe_bytes": args.copy_probe, "repeats": 3}), flush=True)
+        return
 
     from mcmatlab_amd.api import MCPlan
     from mcmatlab_amd.workloads import layered_tissue_mc

@@ -29,6 +29,7 @@
 #include <cmath>
 #include <cstring>
 #include <memory>
+#include <type_traits>
 #include <vector>
 
 #include "mcm_common.h"
@@ -318,6 +319,196 @@ __global__ void __launch_bounds__(kHeatTPB) heat_sweep_z(HeatArgs<RT> a) {
   fold_lds_max(L.maxKey, a.maxKey, a.nM);
 }
 
+// ---------------------------------------------------------------------------
+// Wavefront-parallel tridiagonal solve (FP64 x-sweep)
+//
+// A system  a[r] x[r-1] + b[r] x[r] + c[r] x[r+1] = d[r]  of up to 64*S rows
+// is spread over the 64 lanes of a wavefront, S consecutive rows per lane
+// (row r = lane * S + s); unused rows are a = c = 0, b = 1, d = 0.  The
+// matrices of the heat step are strictly diagonally dominant, so no pivoting
+// is needed.  Partition method:
+//   1. each lane eliminates its interior rows 1..S-2, expressing them through
+//      its own first and last unknowns;
+//   2. the first unknown of each lane is eliminated as well, leaving one
+//      tridiagonal equation per lane in the lanes' last unknowns U;
+//   3. that 64-row system is solved by parallel cyclic reduction, six
+//      steps of lane shuffles;
+//   4. each lane recovers its first and interior unknowns from U.
+// Same solution as the sequential Thomas recurrence up to rounding.
+__device__ inline double lane_shfl(double v, int src) { return __shfl(v, src < 0 ? 0 : (src > 63 ? 63 : src), 64); }
+
+template <int S>
+__device__ inline void wave_tridiag(const double (&a)[S], const double (&b)[S], const double (&c)[S], const double (&d)[S],
+                                    double (&x)[S]) {
+  const int lane = __lane_id();
+  double al[S], be[S], ga[S], de[S];
+  double A1 = a[0], B1 = b[0], C1 = c[0], D1 = d[0];
+  double A2 = a[S - 1], B2 = b[S - 1], C2 = c[S - 1], D2 = d[S - 1];
+  if constexpr (S >= 3) {
+    // forward elimination of the interior, first unknown kept symbolic
+    al[1] = a[1];
+    be[1] = b[1];
+    ga[1] = c[1];
+    de[1] = d[1];
+#pragma unroll
+    for (int s = 2; s <= S - 2; ++s) {
+      const double r = a[s] / be[s - 1];
+      al[s] = -r * al[s - 1];
+      be[s] = b[s] - r * ga[s - 1];
+      ga[s] = c[s];
+      de[s] = d[s] - r * de[s - 1];
+    }
+    // backward elimination, last unknown kept symbolic
+#pragma unroll
+    for (int s = S - 3; s >= 1; --s) {
+      const double r = ga[s] / be[s + 1];
+      al[s] -= r * al[s + 1];
+      ga[s] = -r * ga[s + 1];
+      de[s] -= r * de[s + 1];
+    }
+    const double r1 = c[0] / be[1];
+    B1 = b[0] - r1 * al[1];
+    C1 = -r1 * ga[1];
+    D1 = d[0] - r1 * de[1];
+    const double r2 = a[S - 1] / be[S - 2];
+    A2 = -r2 * al[S - 2];
+    B2 = b[S - 1] - r2 * ga[S - 2];
+    D2 = d[S - 1] - r2 * de[S - 2];
+  }
+  // one equation per lane in U = last unknowns:  pa U[l-1] + pb U[l] + pc U[l+1] = pd
+  double pa, pb, pc, pd;
+  if constexpr (S == 1) {
+    pa = A1;
+    pb = B1;
+    pc = C1;
+    pd = D1;
+  } else {
+    // shuffles run in every lane (a lane reading from a lane that is
+    // inactive at the shuffle gets 0), then the edge lanes are masked
+    const bool hasNext = lane < 63;
+    const double sA = lane_shfl(A1, lane + 1), sB = lane_shfl(B1, lane + 1);
+    const double sC = lane_shfl(C1, lane + 1), sD = lane_shfl(D1, lane + 1);
+    const double nA = hasNext ? sA : 0.0, nB = hasNext ? sB : 1.0, nC = hasNext ? sC : 0.0, nD = hasNext ? sD : 0.0;
+    pa = -A2 * A1 / B1;
+    pb = B2 - A2 * C1 / B1 - C2 * nA / nB;
+    pc = -C2 * nC / nB;
+    pd = D2 - A2 * D1 / B1 - C2 * nD / nB;
+  }
+#pragma unroll
+  for (int h = 1; h < 64; h <<= 1) {
+    const bool lo = lane >= h, hi = lane + h < 64;
+    const double sam = lane_shfl(pa, lane - h), sbm = lane_shfl(pb, lane - h);
+    const double scm = lane_shfl(pc, lane - h), sdm = lane_shfl(pd, lane - h);
+    const double sap = lane_shfl(pa, lane + h), sbp = lane_shfl(pb, lane + h);
+    const double scp = lane_shfl(pc, lane + h), sdp = lane_shfl(pd, lane + h);
+    const double am = lo ? sam : 0.0, bm = lo ? sbm : 1.0, cm = lo ? scm : 0.0, dm = lo ? sdm : 0.0;
+    const double ap = hi ? sap : 0.0, bp = hi ? sbp : 1.0, cp = hi ? scp : 0.0, dp = hi ? sdp : 0.0;
+    const double k1 = pa / bm, k2 = pc / bp;
+    pb = pb - k1 * cm - k2 * ap;
+    pd = pd - k1 * dm - k2 * dp;
+    pa = -k1 * am;
+    pc = -k2 * cp;
+  }
+  const double U = pd / pb;
+  if constexpr (S == 1) {
+    x[0] = U;
+  } else {
+    const double sU = lane_shfl(U, lane - 1);
+    const double Up = lane > 0 ? sU : 0.0;
+    const double first = (D1 - A1 * Up - C1 * U) / B1;
+    x[0] = first;
+    x[S - 1] = U;
+    if constexpr (S >= 3) {
+#pragma unroll
+      for (int s = 1; s <= S - 2; ++s) x[s] = (de[s] - al[s] * first - ga[s] * U) / be[s];
+    }
+  }
+}
+
+// substeps 1a + 1b in FP64 with one wavefront per (y, z) x-line: lane l holds
+// voxels x = l*S .. l*S+S-1, so every load of the wavefront is contiguous.
+// The right-hand side is the full 7-point explicit stencil (x half-weighted)
+// plus the source, in the expression order of the sequential kernel.
+template <int S>
+__global__ void __launch_bounds__(kHeatTPB) heat_sweep_x_wave(HeatArgs<double> a) {
+  __shared__ double sc[kLdsCoef];
+  const int nc = a.nM * a.nM * 3;
+  if (nc <= kLdsCoef)
+    for (int k = threadIdx.x; k < nc; k += blockDim.x) sc[k] = a.c[k];
+  __syncthreads();
+  const double* __restrict__ cT = nc <= kLdsCoef ? sc : a.c;
+  const int lane = __lane_id();
+  const int line = blockIdx.x * (kHeatTPB / 64) + (threadIdx.x >> 6);
+  const int nx = a.nx, ny = a.ny, nz = a.nz, nxy = nx * ny, nM = a.nM;
+  if (line >= ny * nz) return;
+  const int y = line % ny, z = line / ny;
+  const int i0 = y * nx + z * nxy;
+  const bool sk = a.sinked;
+  const bool lf = sk && (y == 0 || y == ny - 1 || z == 0 || z == nz - 1);
+  const double* __restrict__ T1 = a.T1;
+  const uint8_t* __restrict__ Mg = a.M;
+  const float* __restrict__ Q = a.Q;
+  const double dt = a.dt;
+  double T[S];
+  int Mv[S];
+#pragma unroll
+  for (int s = 0; s < S; ++s) {
+    const int k = lane * S + s;
+    T[s] = k < nx ? T1[i0 + k] : 0.0;
+    Mv[s] = k < nx ? (int)Mg[i0 + k] : 0;
+  }
+  const double Tl = lane_shfl(T[S - 1], lane - 1), Tr = lane_shfl(T[0], lane + 1);
+  const int Ml = __shfl(Mv[S - 1], lane > 0 ? lane - 1 : 0, 64), Mr = __shfl(Mv[0], lane < 63 ? lane + 1 : 63, 64);
+  double ca[S], cb[S], cc_[S], cd[S], xs[S];
+#pragma unroll
+  for (int s = 0; s < S; ++s) {
+    const int k = lane * S + s;
+    ca[s] = 0.0;
+    cb[s] = 1.0;
+    cc_[s] = 0.0;
+    cd[s] = 0.0;
+    if (k >= nx) continue;
+    const int i = i0 + k;
+    const int m = Mv[s];
+    const double TL = s > 0 ? T[s - 1] : Tl, TR = s < S - 1 ? T[s + 1] : Tr;
+    const int ML = s > 0 ? Mv[s - 1] : Ml, MR = s < S - 1 ? Mv[s + 1] : Mr;
+    const bool fz = lf || (sk && (k == 0 || k == nx - 1));
+    if (fz) {
+      cd[s] = T[s];
+      continue;
+    }
+    double dTdt = a.lightsOn ? (double)Q[i] : 0.0;
+    if (k != 0) dTdt += (TL - T[s]) * coef(cT, nM, m, ML, 0) / 2;
+    if (k != nx - 1) dTdt += (TR - T[s]) * coef(cT, nM, m, MR, 0) / 2;
+    if (y != 0) dTdt += (T1[i - nx] - T[s]) * coef(cT, nM, m, (int)Mg[i - nx], 1);
+    if (y != ny - 1) dTdt += (T1[i + nx] - T[s]) * coef(cT, nM, m, (int)Mg[i + nx], 1);
+    if (z != 0) dTdt += (T1[i - nxy] - T[s]) * coef(cT, nM, m, (int)Mg[i - nxy], 2);
+    if (z != nz - 1) dTdt += (T1[i + nxy] - T[s]) * coef(cT, nM, m, (int)Mg[i + nxy], 2);
+    cd[s] = T[s] + dt * dTdt;
+    if (k < nx - 1) {
+      const double q = dt / 2 * coef(cT, nM, m, MR, 0);
+      cb[s] += q;
+      cc_[s] = -q;
+    }
+    if (k > 0) {
+      const double q = dt / 2 * coef(cT, nM, m, ML, 0);
+      cb[s] += q;
+      ca[s] = -q;
+    }
+  }
+  wave_tridiag<S>(ca, cb, cc_, cd, xs);
+  double* __restrict__ T2 = a.T2;
+#pragma unroll
+  for (int s = 0; s < S; ++s) {
+    const int k = lane * S + s;
+    if (k < nx) T2[i0 + k] = xs[s];
+  }
+}
+
+// rows per lane of the wavefront x-solve for a line of n voxels (0: use the
+// sequential lane-per-line kernel)
+inline int wave_rows(int n) { return n <= 64 ? 1 : n <= 128 ? 2 : n <= 256 ? 4 : n <= 512 ? 8 : 0; }
+
 constexpr int kExplicitZ = 8;
 
 // Forward Euler step (no reference counterpart; same neighbour order and
@@ -492,6 +683,25 @@ int record_sensors(mcm_heat_plan* P, int64_t col) {
   return MCM_OK;
 }
 
+// x-sweep launch: the wavefront solver for FP64 lines of up to 512 voxels,
+// else the sequential lane-per-line kernel (FP32 keeps the reference's
+// sequential recurrence).
+template <typename RT>
+void launch_sweep_x(mcm_heat_plan* P, const HeatArgs<RT>& a, unsigned gx) {
+  if constexpr (std::is_same<RT, double>::value) {
+    const int S = wave_rows(P->nx);
+    const unsigned gw = (unsigned)(((int64_t)P->ny * P->nz + kHeatTPB / 64 - 1) / (kHeatTPB / 64));
+    switch (S) {
+      case 1: hipLaunchKernelGGL(heat_sweep_x_wave<1>, dim3(gw), dim3(kHeatTPB), 0, P->stream, a); return;
+      case 2: hipLaunchKernelGGL(heat_sweep_x_wave<2>, dim3(gw), dim3(kHeatTPB), 0, P->stream, a); return;
+      case 4: hipLaunchKernelGGL(heat_sweep_x_wave<4>, dim3(gw), dim3(kHeatTPB), 0, P->stream, a); return;
+      case 8: hipLaunchKernelGGL(heat_sweep_x_wave<8>, dim3(gw), dim3(kHeatTPB), 0, P->stream, a); return;
+      default: break;
+    }
+  }
+  hipLaunchKernelGGL(heat_sweep_x<RT>, dim3(gx), dim3(kHeatTPB), 0, P->stream, a);
+}
+
 template <typename RT>
 int advance(mcm_heat_plan* P, int64_t steps, double dt, int lightsOn) {
   int rc = ensure_sensor_capacity(P, P->stepsDone + steps + 1);
@@ -510,7 +720,7 @@ int advance(mcm_heat_plan* P, int64_t steps, double dt, int lightsOn) {
       const unsigned ge = (unsigned)(((int64_t)P->nx * P->ny * ((P->nz + kExplicitZ - 1) / kExplicitZ) + tpb - 1) / tpb);
       hipLaunchKernelGGL(heat_explicit<RT>, dim3(ge), dim3(tpb), 0, P->stream, a);
     } else {
-      hipLaunchKernelGGL(heat_sweep_x<RT>, dim3(gx), dim3(tpb), 0, P->stream, a);
+      launch_sweep_x<RT>(P, a, gx);
       hipLaunchKernelGGL(heat_sweep_y<RT>, dim3(gy), dim3(tpb), 0, P->stream, a);
       hipLaunchKernelGGL(heat_sweep_z<RT>, dim3(gz), dim3(tpb), 0, P->stream, a);
     }

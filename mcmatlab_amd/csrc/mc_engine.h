@@ -9,7 +9,7 @@
 // Tally writes go through Sink:
 //   deposit(j, w)   absorbed weight into voxel j (the sink may merge
 //                   consecutive deposits into one voxel), flush() at death
-//   image(bin, w), farField(bin, w), edge(face, idx, w), collected()
+//   image_add(bin, w), farField(bin, w), edge(face, idx, w), collected()
 #pragma once
 #include "mc_photon.h"
 
@@ -28,7 +28,7 @@ struct PacketEngine {
   float pendAbsorb;
   int pendJ, pendM;
 
-  __host__ __device__ void init(uint32_t k0, uint32_t k1, uint32_t stream) {
+  MCM_FN void init(uint32_t k0, uint32_t k1, uint32_t stream) {
     R.key0 = k0;
     R.key1 = k1;
     R.stream = stream;
@@ -40,7 +40,7 @@ struct PacketEngine {
 
   // launchPhoton + getNewVoxelProperties for packet `id` (MCmatlab.c:247-248).
   // False when the launch fails 10^6 times (the reference aborts the run).
-  __host__ __device__ bool start(uint64_t id, const McParams& p, const Medium* med, const uint8_t* cdf) {
+  MCM_FN bool start(uint64_t id, const McParams& p, const Medium* med, const uint8_t* cdf) {
     packet = id;
     R.start(id);
     if (!launch(P, p, med, R)) {
@@ -53,7 +53,7 @@ struct PacketEngine {
   }
 
   template <class Sink>
-  __host__ __device__ void death(const McParams& p, const Medium* med, const uint8_t* cdf, Sink& sink) {
+  MCM_FN void death(const McParams& p, const Medium* med, const uint8_t* cdf, Sink& sink) {
     sink.flush();
     if (CRITERIA && !replay && p.evalAtEnd && criteria_met(P, p)) {
       // end-of-life deposition (MCmatlab.c:262-264): replay the packet and
@@ -70,7 +70,7 @@ struct PacketEngine {
   // deposit of the sub-step, then checkEscape + getNewVoxelProperties
   // (propagatePhoton :1095-1110, MCmatlab.c:254-257)
   template <class Sink>
-  __host__ __device__ void finish_substep(float absorb, const McParams& p, const Medium* med, const uint8_t* cdf, Sink& sink) {
+  MCM_FN void finish_substep(float absorb, const McParams& p, const Medium* med, const uint8_t* cdf, Sink& sink) {
     if (P.insideVolume && p.calcNFR) {
       if (CRITERIA && replay) sink.deposit(P.j, (double)(replayW * absorb));
       else if (!(CRITERIA && p.evalAtEnd) && criteria_met(P, p)) sink.deposit(P.j, (double)absorb);
@@ -82,7 +82,7 @@ struct PacketEngine {
           if (p.useLC) {
             const int b = collector_bin(P, p);
             if (b >= 0 && criteria_met(P, p)) {
-              sink.image(b, (double)P.weight);
+              sink.image_add(b, (double)P.weight);
               sink.collected();
             }
           }
@@ -101,7 +101,7 @@ struct PacketEngine {
   }
 
   template <class Sink>
-  __host__ __device__ void do_move(const McParams& p, const Medium* med, const uint8_t* cdf, Sink& sink) {
+  MCM_FN void do_move(const McParams& p, const Medium* med, const uint8_t* cdf, Sink& sink) {
     if (P.substeps >= kMaxSubsteps) {
       sink.runaway();
       P.alive = false;
@@ -120,13 +120,13 @@ struct PacketEngine {
   }
 
   template <class Sink>
-  __host__ __device__ void do_fresnel(const McParams& p, const Medium* med, const uint8_t* cdf, Sink& sink) {
+  MCM_FN void do_fresnel(const McParams& p, const Medium* med, const uint8_t* cdf, Sink& sink) {
     fresnel(P, p, med, pendJ, pendM, R);
     finish_substep(pendAbsorb, p, med, cdf, sink);
   }
 
   template <class Sink>
-  __host__ __device__ void do_scatter(const McParams& p, const Medium* med, const uint8_t* cdf, Sink& sink) {
+  MCM_FN void do_scatter(const McParams& p, const Medium* med, const uint8_t* cdf, Sink& sink) {
     roulette(P, R);
     if (!P.alive) {
       death(p, med, cdf, sink);

@@ -74,30 +74,41 @@ __device__ inline unsigned long long bcast64(unsigned long long v, int src) {
 __device__ inline void fadd(double* p, double v) { unsafeAtomicAdd(p, v); }
 
 // Tally sink of the GPU kernel: FP64 hardware atomics; consecutive deposits
-// into the same voxel are merged in a register first.
+// into the same voxel are merged in a register first.  Holds copies of the
+// tally pointers (not a pointer to the launch parameters, whose address must
+// not escape or the parameters are copied to scratch).
 struct DeviceSink {
-  const McParams* p;
+  double* NFR;
+  double* image;
+  double* FF;
+  double *ni0, *ni1, *ni2, *ni3, *ni4, *ni5;
   double acc;
   int accJ;
   unsigned long long nCollected;
   unsigned nRunaway;
+  __device__ explicit DeviceSink(const McParams& p)
+      : NFR(p.NFR), image(p.image), FF(p.FF), ni0(p.NI[0]), ni1(p.NI[1]), ni2(p.NI[2]), ni3(p.NI[3]), ni4(p.NI[4]),
+        ni5(p.NI[5]), acc(0.0), accJ(-1), nCollected(0), nRunaway(0) {}
   __device__ void deposit(int j, double v) {
     if (j == accJ) {
       acc += v;
     } else {
-      if (accJ >= 0) fadd(&p->NFR[accJ], acc);
+      if (accJ >= 0) fadd(&NFR[accJ], acc);
       accJ = j;
       acc = v;
     }
   }
   __device__ void flush() {
-    if (accJ >= 0) fadd(&p->NFR[accJ], acc);
+    if (accJ >= 0) fadd(&NFR[accJ], acc);
     accJ = -1;
     acc = 0.0;
   }
-  __device__ void image(int b, double w) { fadd(&p->image[b], w); }
-  __device__ void farField(int b, double w) { fadd(&p->FF[b], w); }
-  __device__ void edge(int f, int i, double w) { fadd(&p->NI[f][i], w); }
+  __device__ void image_add(int b, double w) { fadd(&image[b], w); }
+  __device__ void farField(int b, double w) { fadd(&FF[b], w); }
+  __device__ void edge(int f, int i, double w) {
+    double* base = f == 0 ? ni0 : f == 1 ? ni1 : f == 2 ? ni2 : f == 3 ? ni3 : f == 4 ? ni4 : ni5;
+    fadd(&base[i], w);
+  }
   __device__ void collected() { ++nCollected; }
   __device__ void runaway() { ++nRunaway; }
 };
@@ -111,12 +122,11 @@ __global__ void __launch_bounds__(256) mc_transport(McParams p, McLaunch lp) {
     sCdf[k] = p.cdfIdx[k];
   }
   __syncthreads();
-  if (!CRITERIA) p.criteriaActive = 0;
 
   const int lane = __lane_id();
   PacketEngine<FRESNEL, CRITERIA> E;
   E.init(lp.key0, lp.key1, lp.stream);
-  DeviceSink sink{&p, 0.0, -1, 0ull, 0u};
+  DeviceSink sink(p);
   unsigned long long nLaunched = 0, nSub = 0, nScat = 0, nIf = 0;
 
   for (;;) {

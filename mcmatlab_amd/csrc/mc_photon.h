@@ -112,19 +112,19 @@ struct MoveResult {
   bool fresnel;  // refractive index changes: interface event pending
 };
 
-__host__ __device__ inline int clampi(int x, int lo, int hi) { return x < lo ? lo : (x > hi ? hi : x); }
+MCM_FN int clampi(int x, int lo, int hi) { return x < lo ? lo : (x > hi ? hi : x); }
 
-__host__ __device__ inline int clamp_axis(float i, int n) { return i < 0.f ? 0 : (i >= (float)n ? n - 1 : (int)floorf(i)); }
+MCM_FN int clamp_axis(float i, int n) { return i < 0.f ? 0 : (i >= (float)n ? n - 1 : (int)floorf(i)); }
 
 // getNewj (MCmatlablib.c:281-285)
-__host__ __device__ inline int voxel_of(const McParams& p, const float* i) {
+MCM_FN int voxel_of(const McParams& p, const float* i) {
   return clamp_axis(i[2], p.n[2]) * (p.n[0] * p.n[1]) + clamp_axis(i[1], p.n[1]) * p.n[0] + clamp_axis(i[0], p.n[0]);
 }
 
 // Unique j in [0, N-1] with D[j] < x <= D[j+1] for a nondecreasing table
 // D[0..N] (the reference's binaryTreeSearch contract, MCmatlablib.c:263-276).
 template <typename T>
-__host__ __device__ inline int table_search(T x, int N, const T* D) {
+MCM_FN int table_search(T x, int N, const T* D) {
   int lo = 0, hi = N;  // invariant: D[lo] < x (for lo > 0) and x <= D[hi]
   while (hi - lo > 1) {
     const int mid = (lo + hi) >> 1;
@@ -135,7 +135,7 @@ __host__ __device__ inline int table_search(T x, int N, const T* D) {
 }
 
 // axisrotate (MCmatlablib.c:251-258): rotate r about unit axis a by angle th
-__host__ __device__ inline void rotate_about(const float* r, const float* a, float th, float* o) {
+MCM_FN void rotate_about(const float* r, const float* a, float th, float* o) {
   float st, ct;
   sincosf(th, &st, &ct);
   const float omc = 1.f - ct;
@@ -144,17 +144,17 @@ __host__ __device__ inline void rotate_about(const float* r, const float* a, flo
   o[2] = (a[2] * a[0] * omc - a[1] * st) * r[0] + (a[2] * a[1] * omc + a[0] * st) * r[1] + (a[2] * a[2] * omc + ct) * r[2];
 }
 
-__host__ __device__ inline float boundary_distance(float i, float u, float d) {
+MCM_FN float boundary_distance(float i, float u, float d) {
   return u != 0.f ? (floorf(i) + (u > 0.f ? 1.f : 0.f) - i) * d / u : INFINITY;
 }
 
-__host__ __device__ inline bool inside_box(const McParams& p, const float* i) {
+MCM_FN bool inside_box(const McParams& p, const float* i) {
   return i[0] < (float)p.n[0] && i[0] >= 0.f && i[1] < (float)p.n[1] && i[1] >= 0.f && i[2] < (float)p.n[2] && i[2] >= 0.f;
 }
 
 // Inside the extended region a packet may wander in for boundary types 0 and 2
 // (KILLRANGE times the cuboid, MCmatlablib.c:687-700, :826-840).
-__host__ __device__ inline bool inside_killrange(const McParams& p, const float* i, bool topOpen) {
+MCM_FN bool inside_killrange(const McParams& p, const float* i, bool topOpen) {
   const float h = kKillRange / 2.0f;
   const bool xy = fabsf(i[0] / (float)p.n[0] - 0.5f) < h && fabsf(i[1] / (float)p.n[1] - 0.5f) < h;
   if (topOpen) return xy && i[2] / (float)p.n[2] - 0.5f < h && i[2] >= 0.f;
@@ -162,7 +162,7 @@ __host__ __device__ inline bool inside_killrange(const McParams& p, const float*
 }
 
 // depositionCriteriaMet (MCmatlablib.c:126-136)
-__host__ __device__ inline bool criteria_met(const Photon& P, const McParams& p) {
+MCM_FN bool criteria_met(const Photon& P, const McParams& p) {
   if (!p.criteriaActive) return true;
   const bool deadKilled = !P.alive && P.kec == 0;
   const uint32_t big = 0xFFFFFFFFu;
@@ -172,14 +172,14 @@ __host__ __device__ inline bool criteria_met(const Photon& P, const McParams& p)
          (p.onlyCollected ? P.kec == 2 : true);
 }
 
-__host__ __device__ inline bool in_count_range(int m, const McParams& p) {
+MCM_FN bool in_count_range(int m, const McParams& p) {
   return (uint32_t)m >= p.minIdx && (uint32_t)m <= p.maxIdx;
 }
 
 // Lambert W lower branch W_{-1}(x), -1/e <= x < 0 (LG01 radius sampling,
 // MCmatlablib.c:586-588): Halley iteration (Corless et al. 1996, eq. 5.12)
 // from the asymptotic start near 0 or the branch-point series near -1/e.
-__host__ __device__ inline float lambert_wm1(float x) {
+MCM_FN float lambert_wm1(float x) {
   if (x >= 0.f) return 0.f;
   const float q = x + 0.36787944117144233f;
   if (q <= 0.f) return -1.f;
@@ -203,7 +203,7 @@ __host__ __device__ inline float lambert_wm1(float x) {
   return w;
 }
 
-__host__ __device__ inline float radial_or_tophat_draw(float code, float width, const float* dist, int L, PacketRng& R) {
+MCM_FN float radial_or_tophat_draw(float code, float width, const float* dist, int L, PacketRng& R) {
   if (code == -1.f) return width * sqrtf(R.next());
   if (code == -2.f) return width * sqrtf(-0.5f * logf(R.next()));
   const float a = R.next();
@@ -213,7 +213,7 @@ __host__ __device__ inline float radial_or_tophat_draw(float code, float width, 
 
 // launchPhoton (MCmatlablib.c:519-732).  Returns false when 10^6 attempts
 // all start outside the allowed region (the reference then aborts the run).
-__host__ __device__ inline bool launch(Photon& P, const McParams& p, const Medium* media, PacketRng& R) {
+MCM_FN bool launch(Photon& P, const McParams& p, const Medium* media, PacketRng& R) {
   P.sameVoxel = false;
   P.weight = 1.f;
   P.kec = 0;
@@ -394,7 +394,7 @@ __host__ __device__ inline bool launch(Photon& P, const McParams& p, const Mediu
 
 // getNewVoxelProperties (MCmatlablib.c:894-904); the refractive index is only
 // changed by refraction.
-__host__ __device__ inline void new_voxel_properties(Photon& P, const McParams& p, const Medium* media, const uint8_t* cdfIdx) {
+MCM_FN void new_voxel_properties(Photon& P, const McParams& p, const Medium* media, const uint8_t* cdfIdx) {
   P.j = voxel_of(p, P.i);
   const int m = p.M[P.j];
   const Medium md = media[m];
@@ -407,7 +407,7 @@ __host__ __device__ inline void new_voxel_properties(Photon& P, const McParams& 
 // Movement half of propagatePhoton (MCmatlablib.c:1022-1047, :1086-1093):
 // step to the next boundary or interaction, absorb continuously.
 template <bool FRESNEL>
-__host__ __device__ inline MoveResult move(Photon& P, const McParams& p, const Medium* media) {
+MCM_FN MoveResult move(Photon& P, const McParams& p, const Medium* media) {
   MoveResult r;
   P.sameVoxel = true;
   ++P.substeps;
@@ -454,7 +454,7 @@ __host__ __device__ inline MoveResult move(Photon& P, const McParams& p, const M
 // (it floors an integer minus 0.5), so the normal is the mean of the unit
 // normals of the up-to-8 voxels around the lower corner of the clamped voxel
 // that share the target voxel's refractive index.
-__host__ __device__ inline void fresnel(Photon& P, const McParams& p, const Medium* media, int jNew, int mNew, PacketRng& R) {
+MCM_FN void fresnel(Photon& P, const McParams& p, const Medium* media, int jNew, int mNew, PacketRng& R) {
   const float riNew = media[mNew].ri;
   const float mu = P.ri / riNew;
   int lo[3];
@@ -491,23 +491,24 @@ __host__ __device__ inline void fresnel(Photon& P, const McParams& p, const Medi
     P.u[0] -= 2.f * nx * cin;
     P.u[1] -= 2.f * ny * cin;
     P.u[2] -= 2.f * nz * cin;
-    if (p.criteriaActive && in_count_range(mNew, p)) ++P.reflections;
   } else {
     const float nc = sqrtf(cos2) - mu * cin;
     P.u[0] = nc * nx + mu * P.u[0];
     P.u[1] = nc * ny + mu * P.u[1];
     P.u[2] = nc * nz + mu * P.u[2];
     P.ri = riNew;
-    if (p.criteriaActive && in_count_range(mNew, p)) {
-      ++P.refractions;
-      ++P.transitions;
-    }
   }
+  // counters updated without branches (branch-local increments of different
+  // fields get merged into a store through a selected pointer)
+  const uint32_t counted = (p.criteriaActive && in_count_range(mNew, p)) ? 1u : 0u;
+  P.reflections += reflect ? counted : 0u;
+  P.refractions += reflect ? 0u : counted;
+  P.transitions += reflect ? 0u : counted;
   for (int a = 0; a < 3; ++a) P.D[a] = boundary_distance(P.i[a], P.u[a], p.d[a]);
 }
 
 // xyztoXYZ (MCmatlablib.c:239-246) with precomputed sin/cos
-__host__ __device__ inline void to_lc_frame(const McParams& p, const float* r, float* o) {
+MCM_FN void to_lc_frame(const McParams& p, const float* r, float* o) {
   o[0] = p.lcSinP * r[0] - p.lcCosP * r[1];
   o[1] = p.lcCosT * p.lcCosP * r[0] + p.lcCosT * p.lcSinP * r[1] - p.lcSinT * r[2];
   o[2] = p.lcSinT * p.lcCosP * r[0] + p.lcSinT * p.lcSinP * r[1] + p.lcCosT * r[2];
@@ -515,7 +516,7 @@ __host__ __device__ inline void to_lc_frame(const McParams& p, const float* r, f
 
 // Light collector hit test (formImage, MCmatlablib.c:737-788).  Marks the
 // packet collected and returns the image bin, or -1.
-__host__ __device__ inline int collector_bin(Photon& P, const McParams& p) {
+MCM_FN int collector_bin(Photon& P, const McParams& p) {
   float U[3];
   to_lc_frame(p, P.u, U);
   if (!(U[2] < 0.f)) return -1;
@@ -550,7 +551,7 @@ __host__ __device__ inline int collector_bin(Photon& P, const McParams& p) {
 }
 
 // formFarField bin (MCmatlablib.c:793-797)
-__host__ __device__ inline int far_field_bin(const Photon& P, const McParams& p) {
+MCM_FN int far_field_bin(const Photon& P, const McParams& p) {
   const float th = (1.f - kEps) * acosf(fminf(1.f, fmaxf(-1.f, P.u[2])));
   const float ph = (1.f - kEps) * (kPi + atan2f(P.u[1], P.u[0]));
   const int r = p.farFieldRes;
@@ -561,7 +562,7 @@ __host__ __device__ inline int far_field_bin(const Photon& P, const McParams& p)
 
 // formEdgeFluxes target (MCmatlablib.c:802-816): face 0..5 = xpos, xneg,
 // ypos, yneg, zpos, zneg, and the index in that face's array; face -1 = none.
-__host__ __device__ inline int edge_flux_bin(const Photon& P, const McParams& p, int* face) {
+MCM_FN int edge_flux_bin(const Photon& P, const McParams& p, int* face) {
   const int nx = p.n[0], ny = p.n[1], nz = p.n[2];
   *face = -1;
   if (p.boundaryType == 1) {
@@ -594,7 +595,7 @@ struct EscapeResult {
   bool edgeFlux;   // died on a boundary: add weight to the NI_* face array
 };
 
-__host__ __device__ inline EscapeResult check_escape(Photon& P, const McParams& p) {
+MCM_FN EscapeResult check_escape(Photon& P, const McParams& p) {
   EscapeResult e;
   e.escaped = false;
   switch (p.boundaryType) {
@@ -627,7 +628,7 @@ __host__ __device__ inline EscapeResult check_escape(Photon& P, const McParams& 
 }
 
 // checkRoulette (MCmatlablib.c:1124-1133)
-__host__ __device__ inline void roulette(Photon& P, PacketRng& R) {
+MCM_FN void roulette(Photon& P, PacketRng& R) {
   if (P.weight < kThreshold) {
     if (R.next() <= kChance) P.weight /= kChance;
     else P.alive = false;
@@ -635,7 +636,7 @@ __host__ __device__ inline void roulette(Photon& P, PacketRng& R) {
 }
 
 // scatterPhoton (MCmatlablib.c:1138-1184)
-__host__ __device__ inline void scatter(Photon& P, const McParams& p, const float* cdfs, int mCur, PacketRng& R) {
+MCM_FN void scatter(Photon& P, const McParams& p, const float* cdfs, int mCur, PacketRng& R) {
   float ct;
   if (isnan(P.g)) {
     const float* D = cdfs + P.cdf * kCdfSize;

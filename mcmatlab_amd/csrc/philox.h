@@ -10,9 +10,14 @@
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
+// Every device function of the transport is force-inlined: an outlined call
+// would pass the packet state and the launch parameters by pointer, which
+// moves them into per-lane scratch memory.
+#define MCM_FN __host__ __device__ inline __attribute__((always_inline))
+
 namespace mcm {
 
-__host__ __device__ inline void philox4x32_10(uint32_t c[4], uint32_t k0, uint32_t k1) {
+MCM_FN void philox4x32_10(uint32_t c[4], uint32_t k0, uint32_t k1) {
 #pragma unroll
   for (int r = 0; r < 10; ++r) {
     const uint64_t p0 = (uint64_t)0xD2511F53u * c[0];
@@ -32,37 +37,38 @@ struct PacketRng {
   uint32_t key0, key1;
   uint32_t plo, phi, stream;
   uint32_t block;
-  uint32_t buf[4];
+  uint32_t b0, b1, b2, b3;  // unused words of the current block, next first
   int have;
 
-  __host__ __device__ void start(uint64_t packet) {
+  MCM_FN void start(uint64_t packet) {
     plo = (uint32_t)packet;
     phi = (uint32_t)(packet >> 32);
     block = 0;
     have = 0;
   }
-  __host__ __device__ uint32_t next_word() {
+  // The block's words are consumed by shifting them down rather than by
+  // indexing, so the state stays in registers.
+  MCM_FN uint32_t next_word() {
     if (have == 0) {
-      buf[0] = block++;
-      buf[1] = plo;
-      buf[2] = phi;
-      buf[3] = stream;
-      philox4x32_10(buf, key0, key1);
+      uint32_t c[4] = {block++, plo, phi, stream};
+      philox4x32_10(c, key0, key1);
+      b0 = c[0];
+      b1 = c[1];
+      b2 = c[2];
+      b3 = c[3];
       have = 4;
     }
-    const int k = 4 - have;
+    const uint32_t w = b0;
+    b0 = b1;
+    b1 = b2;
+    b2 = b3;
     --have;
-    // select without dynamic register indexing
-    uint32_t w = buf[0];
-    w = k == 1 ? buf[1] : w;
-    w = k == 2 ? buf[2] : w;
-    w = k == 3 ? buf[3] : w;
     return w;
   }
   // uniform float on (0, 1]
-  __host__ __device__ float next() { return (float)((next_word() >> 8) + 1u) * (1.0f / 16777216.0f); }
+  MCM_FN float next() { return (float)((next_word() >> 8) + 1u) * (1.0f / 16777216.0f); }
   // uniform double on (0, 1] with 53 random bits (3D-source voxel choice)
-  __host__ __device__ double next53() {
+  MCM_FN double next53() {
     const uint64_t hi = next_word() >> 5, lo = next_word() >> 6;  // 27 + 26 bits
     return ((double)((hi << 26) | lo) + 1.0) * (1.0 / 9007199254740992.0);
   }

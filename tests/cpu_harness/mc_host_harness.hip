@@ -24,7 +24,7 @@ struct HostSink {
   uint64_t collectedN = 0, runawayN = 0;
   void deposit(int j, double v) { p->NFR[j] += v; }
   void flush() {}
-  void image(int b, double w) { p->image[b] += w; }
+  void image_add(int b, double w) { p->image[b] += w; }
   void farField(int b, double w) { p->FF[b] += w; }
   void edge(int f, int i, double w) { p->NI[f][i] += w; }
   void collected() { ++collectedN; }

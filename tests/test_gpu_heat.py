@@ -123,3 +123,32 @@ def test_gpu_heat_invalid_medium_is_an_error(api):
     P["M"][0, 0, 0] = 7  # beyond nM - 1
     with pytest.raises(MCMError):
         api.finiteElementHeatPropagator(T0, Om0, P)
+
+
+def line_case(nx, ny, nz, bt, seed=0, steps=6):
+    """Random 4-medium block with an nx-voxel x extent (the FP64 x-solve runs
+    one wavefront per x-line for nx <= 512 and the lane-per-line recurrence
+    above that)."""
+    rng = np.random.default_rng(seed)
+    d = 0.1 / 64
+    M = rng.integers(0, 4, (nx, ny, nz)).astype(np.uint8)
+    T0 = (37 + 10 * rng.random((nx, ny, nz))).astype(np.float32)
+    Om0 = np.zeros((nx, ny, nz), np.float32)
+    P = {"M": M, "A": np.array([0, 0, 0, 7.6e66]), "E": np.array([0, 0, 0, 422.5e3]),
+         "dTdtperdeltaT": heat_coefficients([1.2e-3, 3.7, 3.9, 3.8], [2.6e-4, 0.0037, 0.0052, 0.0045], d, d, d),
+         "dTdt_abs": (2e4 * rng.random((nx, ny, nz))).astype(np.float32), "useAllCPUs": True, "heatBoundaryType": bt,
+         "tempSensorCornerIdxs": np.zeros(0), "tempSensorInterpWeights": np.zeros((0, 3)), "lightsOn": True,
+         "steps": steps, "dt": 2e-5}
+    return T0, Om0, P
+
+
+@pytest.mark.parametrize("bt", [0, 1])
+@pytest.mark.parametrize("nx", [1, 64, 65, 129, 300, 512, 513])
+def test_gpu_heat_x_line_lengths(api, nx, bt):
+    print(f"[gpu-heat] x-line {nx} bt {bt}", flush=True)
+    T0, Om0, P = line_case(nx, 3, 4, bt, seed=nx)
+    api.finiteElementHeatPropagator(T0, Om0, P, precision="fp64")
+    T64, Om64 = api.finiteElementHeatPropagator.last_state
+    ref = OL.oracle_heat(T0, Om0, P, variant=OL.HEAT_FP64_DGADI)
+    assert rel(T64, ref["T_f64"]) <= 1e-9
+    assert rel(Om64, ref["Omega_f64"]) <= 1e-9
