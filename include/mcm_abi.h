@@ -200,6 +200,27 @@ int mcm_mc_plan_tallies(mcm_mc_plan *plan, const double **d_deposit, uint64_t *n
 int mcm_mc_plan_counters(mcm_mc_plan *plan, double counters[8]);
 int mcm_mc_plan_destroy(mcm_mc_plan *plan);
 
+/* Batched launch (no reference counterpart: the reference runs one
+ * wavelength after another, MCmatlab.c:619-747).  Item i asks for nPhotons
+ * packets of wavelength iL drawn from Philox stream (seed, stream); an item
+ * reproduces mcm_mc_plan_simulate(plan, iL, nPhotons, seed, 0, .) when
+ * stream == iL.  All items run in one sequence of persistent launches whose
+ * lanes take packets of any item, so small items do not leave the GPU idle. */
+typedef struct mcm_mc_item {
+  int64_t iL;          /* wavelength (column of mediaProperties) */
+  uint64_t nPhotons;   /* packets of this item */
+  uint64_t seed;       /* Philox key */
+  uint32_t stream;     /* Philox stream id */
+  uint32_t reserved;   /* 0 */
+} mcm_mc_item;
+/* Runs nItems (1..256) items; outs[i] receives item i's outputs normalised
+ * like one wavelength of mcm_mc_run (NFR [nx,ny,nz], image, farField and NI
+ * faces sized as mcm_mc_output_sizes reports divided by nL), nPhotons,
+ * nPhotonsCollected, nThreads and the batch's simulationTime.  Deposition
+ * criteria, 3D sources and wavelength-dependent refractive indices are
+ * MCM_ERR_UNSUPPORTED; nItems * nM must not exceed 2048. */
+int mcm_mc_batch_run(mcm_mc_plan *plan, const mcm_mc_item *items, int64_t nItems, mcm_mc_output *outs);
+
 /* ---- heat ---- */
 int mcm_heat_run(const mcm_heat_input *in, mcm_heat_output *out);
 

@@ -256,12 +256,18 @@ def mc_output_sizes(shape, nL, boundaryType, calcNFR, farFieldRes, useLightColle
     return out
 
 
-class MCOutputs:
-    """Caller-owned output buffers for one mcm_mc_output."""
+class mcm_mc_item(C.Structure):
+    _fields_ = [("iL", C.c_int64), ("nPhotons", C.c_uint64), ("seed", C.c_uint64), ("stream", C.c_uint32),
+                ("reserved", C.c_uint32)]
 
-    def __init__(self, inp: MCInputs):
+
+class MCOutputs:
+    """Caller-owned output buffers for one mcm_mc_output (for `nL`
+    wavelengths; default: the input's)."""
+
+    def __init__(self, inp: MCInputs, nL=None):
         s = inp.struct
-        dims = mc_output_sizes(inp.shape, inp.nL, int(s.boundaryType), bool(s.calcNFR), int(s.farFieldRes),
+        dims = mc_output_sizes(inp.shape, inp.nL if nL is None else nL, int(s.boundaryType), bool(s.calcNFR), int(s.farFieldRes),
                                bool(s.useLightCollector), s.LC_res if math.isfinite(s.LC_res) else 0,
                                s.LC_nTimeBins if math.isfinite(s.LC_nTimeBins) else 0,
                                bool(s.sourceDistribution))

@@ -119,6 +119,23 @@ class MCPlan:
         keys = ("launched", "collected", "substeps", "scatterings", "interfaces", "runaway", "aborted", "lanes")
         return dict(zip(keys, buf[:]))
 
+    def batch(self, items):
+        """mcm_mc_batch_run: items = [(iL, nPhotons, seed, stream), ...] run
+        in one sequence of persistent launches; returns one output dict per
+        item (one wavelength each, normalised like MCmatlab)."""
+        n = len(items)
+        arr = (abi.mcm_mc_item * n)()
+        for k, (iL, nph, seed, stream) in enumerate(items):
+            arr[k].iL, arr[k].nPhotons, arr[k].seed, arr[k].stream, arr[k].reserved = int(iL), int(nph), int(seed), int(stream), 0
+        outs = [abi.MCOutputs(self._inp, nL=1) for _ in range(n)]
+        structs = (abi.mcm_mc_output * n)(*[o.struct for o in outs])
+        check(lib().mcm_mc_batch_run(self.h, arr, n, structs))
+        res = []
+        for o, st in zip(outs, structs):
+            o.struct = st
+            res.append(o.result())
+        return res
+
     def close(self):
         if self.h:
             lib().mcm_mc_plan_destroy(self.h)

@@ -10,6 +10,9 @@
 //   deposit(j, w)   absorbed weight into voxel j (the sink may merge
 //                   consecutive deposits into one voxel), flush() at death
 //   image_add(bin, w), farField(bin, w), edge(face, idx, w), collected()
+//   or, when Sink::kDeferEscapes, escape(P, escaped, edgeFlux, slot): the packet
+//   is recorded and those tallies are formed later (only without deposition
+//   criteria, which need the collector's verdict at death)
 #pragma once
 #include "mc_photon.h"
 
@@ -22,6 +25,7 @@ struct PacketEngine {
   Photon P;
   PacketRng R;
   uint64_t packet;
+  uint32_t slot;  // index of the packet within the current launch (escape record slot)
   int state;
   bool replay;
   float replayW;
@@ -52,6 +56,20 @@ struct PacketEngine {
     return true;
   }
 
+  // Same as start(), from the launch kernel's record of packet `id`.
+  MCM_FN bool start_from(uint64_t id, const LaunchRecord& L, const McParams& p, const Medium* med, const uint8_t* cdf) {
+    packet = id;
+    R.start(id);
+    if (!unpack_launch(L, P, R)) {
+      state = ST_FETCH;
+      return false;
+    }
+    set_crossing_lengths(P, p);
+    new_voxel_properties(P, p, med, cdf);
+    state = P.stepLeft > 0.f ? ST_MOVE : ST_SCATTER;
+    return true;
+  }
+
   template <class Sink>
   MCM_FN void death(const McParams& p, const Medium* med, const uint8_t* cdf, Sink& sink) {
     sink.flush();
@@ -77,7 +95,9 @@ struct PacketEngine {
     }
     if (!P.sameVoxel) {
       const EscapeResult e = check_escape(P, p);
-      if (!replay) {
+      if constexpr (Sink::kDeferEscapes) {
+        if ((e.escaped && (p.useLC || p.farFieldRes)) || e.edgeFlux) sink.escape(P, e.escaped, e.edgeFlux, slot);
+      } else if (!replay) {
         if (e.escaped) {
           if (p.useLC) {
             const int b = collector_bin(P, p);
@@ -136,5 +156,40 @@ struct PacketEngine {
     state = P.stepLeft > 0.f ? ST_MOVE : ST_SCATTER;
   }
 };
+
+// The tallies of a recorded escape (the inline code of finish_substep with
+// the deposition criteria inactive): light collector image, far field and
+// edge fluxes.
+template <class Sink>
+MCM_FN void tally_escape(const EscapeRecord& E, const McParams& p, Sink& sink) {
+  Photon P;
+  P.i[0] = E.a.x;
+  P.i[1] = E.a.y;
+  P.i[2] = E.a.z;
+  P.weight = E.a.w;
+  P.u[0] = E.b.x;
+  P.u[1] = E.b.y;
+  P.u[2] = E.b.z;
+  P.time = E.b.w;
+  P.ri = E.c.x;
+  P.alive = false;
+  P.kec = 0;
+  const int flags = (int)E.c.y;
+  if (flags & 1) {
+    if (p.useLC) {
+      const int b = collector_bin(P, p);
+      if (b >= 0) {
+        sink.image_add(b, (double)P.weight);
+        sink.collected();
+      }
+    }
+    if (p.farFieldRes) sink.farField(far_field_bin(P, p), (double)P.weight);
+  }
+  if (flags & 2) {
+    int face;
+    const int idx = edge_flux_bin(P, p, &face);
+    if (face >= 0) sink.edge(face, idx, (double)P.weight);
+  }
+}
 
 }  // namespace mcm

@@ -38,6 +38,7 @@
 #include <cstdlib>
 #include <cstring>
 #include <memory>
+#include <type_traits>
 #include <vector>
 
 #include "mc_engine.h"
@@ -48,7 +49,7 @@
 namespace mcm {
 
 struct McCounters {
-  unsigned long long nextPacket;
+  unsigned long long nextPacket;  // reset before every launch
   unsigned long long launched;
   unsigned long long collected;
   unsigned long long substeps;
@@ -66,6 +67,40 @@ struct McLaunch {
   McCounters* ctr;
 };
 
+// Work items of a batched launch (nItems == 0: a plain launch).  Item i
+// owns the global packet indices cum[i] .. cum[i+1]-1; packet g of item i is
+// packet g - cum[i] of Philox stream (key[2i], key[2i+1], stream[i]), so an
+// item reproduces a plain launch of the same wavelength, seed and stream.
+// Media tables are per item ([nItems * nM]); tallies are per item slices.
+constexpr int kBatchMaxItems = 256;
+constexpr int kBatchMaxMedia = 2048;  // nItems * nM entries staged in LDS
+
+struct McBatch {
+  int nItems, nM;
+  const unsigned long long* cum;  // [nItems + 1]
+  const uint32_t* key;            // [2 * nItems]
+  const uint32_t* stream;         // [nItems]
+  const Medium* media;            // [nItems * nM]
+  const uint8_t* cdfIdx;          // [nItems * nM]
+  int64_t L, nImage, nFF, nNI[6];
+  double* NFR;                    // [nItems * L]
+  double* image;                  // [nItems * nImage]
+  double* FF;                     // [nItems * nFF]
+  double* NI[6];                  // [nItems * nNI[f]]
+  unsigned long long* collected;  // [nItems]
+};
+
+// item owning global packet index g: the last i with cum[i] <= g
+__device__ inline int batch_item(unsigned long long g, const unsigned long long* cum, int n) {
+  int lo = 0, hi = n;  // cum[lo] <= g < cum[hi]
+  while (hi - lo > 1) {
+    const int mid = (lo + hi) >> 1;
+    if (cum[mid] <= g) lo = mid;
+    else hi = mid;
+  }
+  return lo;
+}
+
 __device__ inline unsigned long long bcast64(unsigned long long v, int src) {
   const unsigned lo = __shfl((unsigned)v, src), hi = __shfl((unsigned)(v >> 32), src);
   return ((unsigned long long)hi << 32) | lo;
@@ -73,11 +108,16 @@ __device__ inline unsigned long long bcast64(unsigned long long v, int src) {
 
 __device__ inline void fadd(double* p, double v) { unsafeAtomicAdd(p, v); }
 
+// Largest number of packets per transport launch (sizes the launch and
+// escape record buffers: 64 + 48 bytes per packet).
+constexpr uint64_t kMaxLaunchPackets = 1ull << 24;
+
 // Tally sink of the GPU kernel: FP64 hardware atomics; consecutive deposits
 // into the same voxel are merged in a register first.  Holds copies of the
 // tally pointers (not a pointer to the launch parameters, whose address must
 // not escape or the parameters are copied to scratch).
 struct DeviceSink {
+  static constexpr bool kDeferEscapes = false;
   double* NFR;
   double* image;
   double* FF;
@@ -113,20 +153,132 @@ struct DeviceSink {
   __device__ void runaway() { ++nRunaway; }
 };
 
-template <bool FRESNEL, bool CRITERIA>
-__global__ void __launch_bounds__(256) mc_transport(McParams p, McLaunch lp) {
-  __shared__ Medium sMed[256];
-  __shared__ uint8_t sCdf[256];
-  for (int k = threadIdx.x; k < 256; k += blockDim.x) {
-    sMed[k] = p.media[k];
-    sCdf[k] = p.cdfIdx[k];
+// Sink of the transport kernel without deposition criteria: a packet dies
+// once, so its escape is written to the record slot of its own launch index
+// (no shared counter) and tallied by mc_tally_escapes after the launch.
+struct DeferSink : DeviceSink {
+  static constexpr bool kDeferEscapes = true;
+  EscapeRecord* rec;
+  int item;
+  __device__ DeferSink(const McParams& p, EscapeRecord* r) : DeviceSink(p), rec(r), item(0) {}
+  __device__ void escape(const Photon& P, bool escaped, bool edgeFlux, uint32_t slot) {
+    EscapeRecord E;
+    pack_escape(P, escaped, edgeFlux, item, E);
+    rec[slot] = E;
+  }
+};
+
+template <class Sink>
+__device__ inline Sink make_sink(const McParams& p, EscapeRecord* r);
+template <>
+__device__ inline DeviceSink make_sink<DeviceSink>(const McParams& p, EscapeRecord*) {
+  return DeviceSink(p);
+}
+template <>
+__device__ inline DeferSink make_sink<DeferSink>(const McParams& p, EscapeRecord* r) {
+  return DeferSink(p, r);
+}
+
+// launchPhoton for packets firstPacket .. firstPacket + nPackets - 1 into
+// rec[0 .. nPackets-1]; clears the packets' escape record slots.  A launch
+// that fails 10^6 times flags the run as aborted and the remaining packets
+// are skipped (the transport kernel does not start any packet of an aborted
+// run).
+template <bool BATCH>
+__global__ void __launch_bounds__(256) mc_launch_packets(McParams p, McLaunch lp, McBatch bt, LaunchRecord* __restrict__ rec,
+                                                         EscapeRecord* __restrict__ esc) {
+  PacketRng R;
+  R.key0 = lp.key0;
+  R.key1 = lp.key1;
+  R.stream = lp.stream;
+  const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
+  for (uint64_t k = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; k < lp.nPackets; k += stride) {
+    if (__hip_atomic_load(&lp.ctr->aborted, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)) break;
+    const Medium* media = p.media;
+    if constexpr (BATCH) {
+      const unsigned long long g = lp.firstPacket + k;
+      const int it = batch_item(g, bt.cum, bt.nItems);
+      R.key0 = bt.key[2 * it];
+      R.key1 = bt.key[2 * it + 1];
+      R.stream = bt.stream[it];
+      R.start(g - bt.cum[it]);
+      media = bt.media + (size_t)it * bt.nM;
+    } else {
+      R.start(lp.firstPacket + k);
+    }
+    Photon P;
+    const bool ok = launch(P, p, media, R);
+    LaunchRecord L;
+    pack_launch(P, R, ok, L);
+    rec[k] = L;
+    esc[k].c = make_float4(0.f, 0.f, 0.f, 0.f);
+    if (!ok) atomicOr(&lp.ctr->aborted, 1u);
+  }
+}
+
+// Light collector, far field and edge-flux tallies of the recorded escapes
+// (slots whose flags are zero belong to packets absorbed inside).
+template <bool BATCH>
+__global__ void __launch_bounds__(256) mc_tally_escapes(McParams p, McLaunch lp, McBatch bt, const EscapeRecord* __restrict__ rec) {
+  DeviceSink sink(p);
+  const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
+  for (uint64_t k = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; k < lp.nPackets; k += stride) {
+    const float4 c = rec[k].c;
+    if (c.y == 0.f) continue;
+    EscapeRecord E;
+    E.a = rec[k].a;
+    E.b = rec[k].b;
+    E.c = c;
+    if constexpr (BATCH) {
+      const int it = (int)c.z;
+      DeviceSink s(p);
+      s.image = bt.image + (size_t)it * bt.nImage;
+      s.FF = bt.FF + (size_t)it * bt.nFF;
+      s.ni0 = bt.NI[0] + (size_t)it * bt.nNI[0];
+      s.ni1 = bt.NI[1] + (size_t)it * bt.nNI[1];
+      s.ni2 = bt.NI[2] + (size_t)it * bt.nNI[2];
+      s.ni3 = bt.NI[3] + (size_t)it * bt.nNI[3];
+      s.ni4 = bt.NI[4] + (size_t)it * bt.nNI[4];
+      s.ni5 = bt.NI[5] + (size_t)it * bt.nNI[5];
+      tally_escape(E, p, s);
+      if (s.nCollected) atomicAdd(&bt.collected[it], s.nCollected);
+    } else {
+      tally_escape(E, p, sink);
+    }
+  }
+  if (sink.nCollected) atomicAdd(&lp.ctr->collected, sink.nCollected);
+}
+
+template <bool FRESNEL, bool CRITERIA, bool BATCH>
+__global__ void __launch_bounds__(256) mc_transport(McParams p, McLaunch lp, McBatch bt, const LaunchRecord* __restrict__ launches,
+                                                    EscapeRecord* __restrict__ escapes) {
+  static_assert(!(BATCH && CRITERIA), "batched launches run without deposition criteria");
+  constexpr int kMed = BATCH ? kBatchMaxMedia : 256;
+  __shared__ Medium sMed[kMed];
+  __shared__ uint8_t sCdf[kMed];
+  __shared__ unsigned long long sCum[BATCH ? kBatchMaxItems + 1 : 1];
+  if constexpr (BATCH) {
+    for (int k = threadIdx.x; k < bt.nItems * bt.nM; k += blockDim.x) {
+      sMed[k] = bt.media[k];
+      sCdf[k] = bt.cdfIdx[k];
+    }
+    for (int k = threadIdx.x; k <= bt.nItems; k += blockDim.x) sCum[k] = bt.cum[k];
+  } else {
+    for (int k = threadIdx.x; k < 256; k += blockDim.x) {
+      sMed[k] = p.media[k];
+      sCdf[k] = p.cdfIdx[k];
+    }
   }
   __syncthreads();
 
   const int lane = __lane_id();
   PacketEngine<FRESNEL, CRITERIA> E;
   E.init(lp.key0, lp.key1, lp.stream);
-  DeviceSink sink(p);
+  using Sink = typename std::conditional<CRITERIA, DeviceSink, DeferSink>::type;
+  Sink sink = make_sink<Sink>(p, escapes);
+  // the media tables of the lane's work item (the launch's in a plain launch)
+  const Medium* med = sMed;
+  const uint8_t* cdf = sCdf;
   unsigned long long nLaunched = 0, nSub = 0, nScat = 0, nIf = 0;
 
   for (;;) {
@@ -149,17 +301,17 @@ __global__ void __launch_bounds__(256) mc_transport(McParams p, McLaunch lp) {
     if (phase == ST_MOVE) {
       if (st == ST_MOVE) {
         ++nSub;
-        E.do_move(p, sMed, sCdf, sink);
+        E.do_move(p, med, cdf, sink);
       }
     } else if (phase == ST_SCATTER) {
       if (st == ST_SCATTER) {
         ++nScat;
-        E.do_scatter(p, sMed, sCdf, sink);
+        E.do_scatter(p, med, cdf, sink);
       }
     } else if (FRESNEL && phase == ST_FRESNEL) {
       if (st == ST_FRESNEL) {
         ++nIf;
-        E.do_fresnel(p, sMed, sCdf, sink);
+        E.do_fresnel(p, med, cdf, sink);
       }
     } else {
       // consecutive packet numbers for the waiting lanes, one atomic per wavefront
@@ -176,7 +328,27 @@ __global__ void __launch_bounds__(256) mc_transport(McParams p, McLaunch lp) {
           E.state = ST_DONE;
         } else {
           E.replay = false;
-          if (E.start(lp.firstPacket + k, p, sMed, sCdf)) {
+          bool ok;
+          E.slot = (uint32_t)k;
+          if constexpr (CRITERIA) {
+            ok = E.start(lp.firstPacket + k, p, med, cdf);
+          } else if constexpr (BATCH) {
+            const unsigned long long g = lp.firstPacket + k;
+            const int it = batch_item(g, sCum, bt.nItems);
+            med = sMed + it * bt.nM;
+            cdf = sCdf + it * bt.nM;
+            E.R.key0 = bt.key[2 * it];
+            E.R.key1 = bt.key[2 * it + 1];
+            E.R.stream = bt.stream[it];
+            sink.NFR = bt.NFR + (size_t)it * bt.L;
+            sink.item = it;
+            const LaunchRecord L = launches[k];
+            ok = E.start_from(g - sCum[it], L, p, med, cdf);
+          } else {
+            const LaunchRecord L = launches[k];
+            ok = E.start_from(lp.firstPacket + k, L, p, med, cdf);
+          }
+          if (ok) {
             ++nLaunched;
           } else {
             atomicOr(&lp.ctr->aborted, 1u);
@@ -312,10 +484,12 @@ struct mcm_mc_plan {
   hipStream_t stream = nullptr;
   McTables T;
   McParams prm;
-  int blocks = 0, threads = 256;
+  int blocks = 0, blocksBatch = 0, threads = 256;
   float moveFrac = 0.5f;
   DevBuf M, normals, media, cdfIdx, cdfs, fpid1, aid1, fpid2, aid2, S, muaNorm;
   DevBuf NFR, image, FF, NI[6], counters, nfrOut;
+  DevBuf launchRec, escapeRec;  // per-launch packet records (no deposition criteria)
+  uint64_t recCap = 0;
   std::vector<double> Shost;   // cumulative 3D source table of the current wavelength
   const float* sourceDistribution = nullptr;
   double power = 0;            // current wavelength's source power
@@ -383,7 +557,30 @@ int select_wavelength(mcm_mc_plan* pl, int64_t iL) {
   return MCM_OK;
 }
 
-int launch_transport(mcm_mc_plan* pl, uint64_t first, uint64_t n, uint64_t seed, uint32_t stream, hipStream_t s) {
+int ensure_records(mcm_mc_plan* pl, uint64_t n) {
+  if (n <= pl->recCap) return MCM_OK;
+  MCM_HIP(hipDeviceSynchronize());  // earlier launches may use the old buffers on any stream
+  for (DevBuf* b : {&pl->launchRec, &pl->escapeRec})
+    if (b->p) {
+      MCM_HIP(hipFree(b->p));
+      b->p = nullptr;
+    }
+  if (hipMalloc(&pl->launchRec.p, (size_t)n * sizeof(LaunchRecord)) != hipSuccess ||
+      hipMalloc(&pl->escapeRec.p, (size_t)n * sizeof(EscapeRecord)) != hipSuccess) {
+    (void)hipGetLastError();
+    pl->recCap = 0;
+    return fail(MCM_ERR_OUT_OF_MEMORY, "Error: Out of memory (packet records for %llu packets)", (unsigned long long)n);
+  }
+  pl->recCap = n;
+  return MCM_OK;
+}
+
+// One launch of n <= kMaxLaunchPackets packets.  Without deposition
+// criteria: launch kernel -> transport -> escape tallies; with criteria the
+// transport launches and tallies inline.  `bt` (batched launches) maps the
+// global packet indices first .. first+n-1 to work items.
+int launch_transport(mcm_mc_plan* pl, uint64_t first, uint64_t n, uint64_t seed, uint32_t stream, hipStream_t s,
+                     const McBatch* bt = nullptr) {
   McLaunch lp;
   lp.firstPacket = first;
   lp.nPackets = n;
@@ -394,22 +591,52 @@ int launch_transport(mcm_mc_plan* pl, uint64_t first, uint64_t n, uint64_t seed,
   lp.ctr = pl->counters.as<McCounters>();
   const bool fr = pl->T.fresnelPossible;
   const bool cr = pl->prm.criteriaActive != 0;
-  const dim3 g(pl->blocks), b(pl->threads);
-  if (fr && cr) hipLaunchKernelGGL((mc_transport<true, true>), g, b, 0, s, pl->prm, lp);
-  else if (fr) hipLaunchKernelGGL((mc_transport<true, false>), g, b, 0, s, pl->prm, lp);
-  else if (cr) hipLaunchKernelGGL((mc_transport<false, true>), g, b, 0, s, pl->prm, lp);
-  else hipLaunchKernelGGL((mc_transport<false, false>), g, b, 0, s, pl->prm, lp);
+  McBatch none;
+  std::memset(&none, 0, sizeof none);
+  const McBatch& B = bt ? *bt : none;
+  const dim3 b(pl->threads);
+  MCM_HIP(hipMemsetAsync(pl->counters.p, 0, sizeof(unsigned long long), s));  // nextPacket
+  if (cr) {
+    const dim3 g(pl->blocks);
+    if (fr) hipLaunchKernelGGL((mc_transport<true, true, false>), g, b, 0, s, pl->prm, lp, B, nullptr, nullptr);
+    else hipLaunchKernelGGL((mc_transport<false, true, false>), g, b, 0, s, pl->prm, lp, B, nullptr, nullptr);
+    MCM_HIP(hipGetLastError());
+    return MCM_OK;
+  }
+  int rc = ensure_records(pl, n);
+  if (rc) return rc;
+  LaunchRecord* L = pl->launchRec.as<LaunchRecord>();
+  EscapeRecord* E = pl->escapeRec.as<EscapeRecord>();
+  const McTables& T = pl->T;
+  bool anyNI = false;
+  for (int f = 0; f < 6; ++f) anyNI = anyNI || T.nNI[f] > 0;
+  const bool tally = T.base.useLC || T.base.farFieldRes || anyNI;
+  if (bt) {
+    const dim3 g(pl->blocksBatch);
+    hipLaunchKernelGGL(mc_launch_packets<true>, g, b, 0, s, pl->prm, lp, B, L, E);
+    if (fr) hipLaunchKernelGGL((mc_transport<true, false, true>), g, b, 0, s, pl->prm, lp, B, L, E);
+    else hipLaunchKernelGGL((mc_transport<false, false, true>), g, b, 0, s, pl->prm, lp, B, L, E);
+    if (tally) hipLaunchKernelGGL(mc_tally_escapes<true>, g, b, 0, s, pl->prm, lp, B, E);
+  } else {
+    const dim3 g(pl->blocks);
+    hipLaunchKernelGGL(mc_launch_packets<false>, g, b, 0, s, pl->prm, lp, B, L, E);
+    if (fr) hipLaunchKernelGGL((mc_transport<true, false, false>), g, b, 0, s, pl->prm, lp, B, L, E);
+    else hipLaunchKernelGGL((mc_transport<false, false, false>), g, b, 0, s, pl->prm, lp, B, L, E);
+    if (tally) hipLaunchKernelGGL(mc_tally_escapes<false>, g, b, 0, s, pl->prm, lp, B, E);
+  }
   MCM_HIP(hipGetLastError());
   return MCM_OK;
 }
 
-int occupancy_blocks(int device, bool fr, bool cr) {
+int occupancy_blocks(int device, bool fr, bool cr, bool batch = false) {
   int per = 0;
-  hipError_t e;
-  if (fr && cr) e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&per, (const void*)mc_transport<true, true>, 256, 0);
-  else if (fr) e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&per, (const void*)mc_transport<true, false>, 256, 0);
-  else if (cr) e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&per, (const void*)mc_transport<false, true>, 256, 0);
-  else e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&per, (const void*)mc_transport<false, false>, 256, 0);
+  const void* k;
+  if (batch) k = fr ? (const void*)mc_transport<true, false, true> : (const void*)mc_transport<false, false, true>;
+  else if (fr && cr) k = (const void*)mc_transport<true, true, false>;
+  else if (fr) k = (const void*)mc_transport<true, false, false>;
+  else if (cr) k = (const void*)mc_transport<false, true, false>;
+  else k = (const void*)mc_transport<false, false, false>;
+  const hipError_t e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&per, k, 256, 0);
   if (e != hipSuccess || per < 1) per = 4;
   return per * cu_count(device);
 }
@@ -488,8 +715,9 @@ int mcm_mc_plan_create(const mcm_mc_input* in, mcm_mc_plan** out) {
   for (int f = 0; f < 6; ++f) p.NI[f] = pl->NI[f].as<double>();
   pl->threads = 256;
   pl->blocks = occupancy_blocks(pl->device, T.fresnelPossible, p.criteriaActive != 0);
+  pl->blocksBatch = occupancy_blocks(pl->device, T.fresnelPossible, false, true);
   if (const char* e = std::getenv("MCM_MC_MOVE_FRAC")) pl->moveFrac = (float)std::atof(e);
-  if (const char* e = std::getenv("MCM_MC_BLOCKS")) pl->blocks = std::max(1, std::atoi(e));
+  if (const char* e = std::getenv("MCM_MC_BLOCKS")) pl->blocks = pl->blocksBatch = std::max(1, std::atoi(e));
   if ((rc = select_wavelength(pl.get(), 0))) return rc;
   MCM_HIP(hipStreamSynchronize(s));
   *out = pl.release();
@@ -505,10 +733,13 @@ int mcm_mc_plan_simulate(mcm_mc_plan* pl, int64_t iL, uint64_t nPhotons, uint64_
   int rc = select_wavelength(pl, iL);
   if (rc) return rc;
   if (s != pl->stream) MCM_HIP(hipStreamSynchronize(pl->stream));
-  // reset the per-launch packet counter; launched/collected accumulate
-  MCM_HIP(hipMemsetAsync(pl->counters.p, 0, sizeof(unsigned long long), s));
-  if (nPhotons == 0) return MCM_OK;
-  return launch_transport(pl, firstPhoton, nPhotons, seed, (uint32_t)iL, s);
+  // launched/collected/... accumulate over launches until normalize
+  for (uint64_t done = 0; done < nPhotons;) {
+    const uint64_t n = std::min<uint64_t>(nPhotons - done, kMaxLaunchPackets);
+    if ((rc = launch_transport(pl, firstPhoton + done, n, seed, (uint32_t)iL, s))) return rc;
+    done += n;
+  }
+  return MCM_OK;
 }
 
 int mcm_mc_plan_tallies(mcm_mc_plan* pl, const double** d_deposit, uint64_t* nPhotons) {
@@ -587,6 +818,131 @@ int mcm_mc_plan_counters(mcm_mc_plan* pl, double* out) {
   out[5] = (double)c.runaway;
   out[6] = (double)c.aborted;
   out[7] = (double)pl->blocks * pl->threads;
+  return MCM_OK;
+}
+
+int mcm_mc_batch_run(mcm_mc_plan* pl, const mcm_mc_item* items, int64_t nItems, mcm_mc_output* outs) {
+  if (!pl || !items || !outs) return fail(MCM_ERR_INVALID_ARGUMENT, "null argument");
+  if (nItems < 1 || nItems > kBatchMaxItems) return fail(MCM_ERR_INVALID_ARGUMENT, "nItems must be 1..%d", kBatchMaxItems);
+  McTables& T = pl->T;
+  if (nItems * T.nM > kBatchMaxMedia) return fail(MCM_ERR_UNSUPPORTED, "nItems * nM exceeds %d", kBatchMaxMedia);
+  if (pl->prm.criteriaActive) return fail(MCM_ERR_UNSUPPORTED, "batched launches run without deposition criteria");
+  if (pl->sourceDistribution) return fail(MCM_ERR_UNSUPPORTED, "batched launches need a beam source");
+  for (int64_t i = 0; i < nItems; ++i) {
+    if (items[i].iL < 0 || items[i].iL >= T.nL) return fail(MCM_ERR_INVALID_ARGUMENT, "item %lld: wavelength out of range", (long long)i);
+    if (T.fresnelPossible)
+      for (int m = 0; m < T.nM; ++m)
+        if (T.media[(size_t)items[i].iL * 256 + m].ri != T.media[(size_t)items[0].iL * 256 + m].ri)
+          return fail(MCM_ERR_UNSUPPORTED, "batched items need the same refractive indices");
+  }
+  MCM_HIP(hipSetDevice(pl->device));
+  int rc = select_wavelength(pl, items[0].iL);  // interface normals and their indices
+  if (rc) return rc;
+  hipStream_t s = pl->stream;
+  const int nI = (int)nItems, nM = T.nM;
+  std::vector<unsigned long long> cum((size_t)nI + 1, 0);
+  std::vector<uint32_t> key(2 * (size_t)nI), strm((size_t)nI);
+  std::vector<Medium> med((size_t)nI * nM);
+  std::vector<uint8_t> cdf((size_t)nI * nM);
+  for (int i = 0; i < nI; ++i) {
+    cum[i + 1] = cum[i] + items[i].nPhotons;
+    key[2 * i] = (uint32_t)items[i].seed;
+    key[2 * i + 1] = (uint32_t)(items[i].seed >> 32);
+    strm[i] = items[i].stream;
+    for (int m = 0; m < nM; ++m) {
+      med[(size_t)i * nM + m] = T.media[(size_t)items[i].iL * 256 + m];
+      cdf[(size_t)i * nM + m] = T.cdfIdx[(size_t)items[i].iL * 256 + m];
+    }
+  }
+  DevBuf dCum, dKey, dStrm, dMed, dCdf, dNFR, dImg, dFF, dNI[6], dColl;
+  if ((rc = upload(dCum, cum.data(), cum.size(), s)) || (rc = upload(dKey, key.data(), key.size(), s)) ||
+      (rc = upload(dStrm, strm.data(), strm.size(), s)) || (rc = upload(dMed, med.data(), med.size(), s)) ||
+      (rc = upload(dCdf, cdf.data(), cdf.size(), s)))
+    return rc;
+  McBatch B;
+  std::memset(&B, 0, sizeof B);
+  B.nItems = nI;
+  B.nM = nM;
+  B.cum = dCum.as<unsigned long long>();
+  B.key = dKey.as<uint32_t>();
+  B.stream = dStrm.as<uint32_t>();
+  B.media = dMed.as<Medium>();
+  B.cdfIdx = dCdf.as<uint8_t>();
+  B.L = T.L;
+  B.nImage = T.nImage;
+  B.nFF = T.nFF;
+  if ((rc = zeros(dNFR, (T.calcNFR ? (size_t)nI * T.L : 1) * sizeof(double), s)) ||
+      (rc = zeros(dImg, (size_t)std::max<int64_t>(nI * T.nImage, 1) * sizeof(double), s)) ||
+      (rc = zeros(dFF, (size_t)std::max<int64_t>(nI * T.nFF, 1) * sizeof(double), s)) ||
+      (rc = zeros(dColl, (size_t)nI * sizeof(unsigned long long), s)))
+    return rc;
+  for (int f = 0; f < 6; ++f) {
+    B.nNI[f] = T.nNI[f];
+    if ((rc = zeros(dNI[f], (size_t)std::max<int64_t>(nI * T.nNI[f], 1) * sizeof(double), s))) return rc;
+    B.NI[f] = dNI[f].as<double>();
+  }
+  B.NFR = dNFR.as<double>();
+  B.image = dImg.as<double>();
+  B.FF = dFF.as<double>();
+  B.collected = dColl.as<unsigned long long>();
+  MCM_HIP(hipMemsetAsync(pl->counters.p, 0, sizeof(McCounters), s));
+  const auto t0 = std::chrono::steady_clock::now();
+  const unsigned long long total = cum[nI];
+  for (unsigned long long done = 0; done < total;) {
+    const uint64_t n = std::min<unsigned long long>(total - done, kMaxLaunchPackets);
+    if ((rc = launch_transport(pl, done, n, 0, 0, s, &B))) return rc;
+    done += n;
+  }
+  MCM_HIP(hipStreamSynchronize(s));
+  const double minutes = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count() / 60.0;
+  McCounters c;
+  MCM_HIP(hipMemcpy(&c, pl->counters.p, sizeof c, hipMemcpyDeviceToHost));
+  pl->stats = c;
+  MCM_HIP(hipMemsetAsync(pl->counters.p, 0, sizeof(McCounters), s));
+  if (c.aborted) return fail(MCM_ERR_INVALID_ARGUMENT, "all packets launch outside the simulation cuboid");
+  std::vector<unsigned long long> coll((size_t)nI);
+  MCM_HIP(hipMemcpy(coll.data(), dColl.p, coll.size() * sizeof(unsigned long long), hipMemcpyDeviceToHost));
+  // per-item normalisation (normalizeDepositionAndResetO, MCmatlablib.c:1186-1250)
+  std::vector<double> h;
+  auto slice = [&](const DevBuf& b, int64_t off, int64_t n, float* dst, double div) -> int {
+    if (n <= 0 || !dst) return MCM_OK;
+    h.resize((size_t)n);
+    MCM_HIP(hipMemcpy(h.data(), b.as<double>() + off, (size_t)n * sizeof(double), hipMemcpyDeviceToHost));
+    for (int64_t k = 0; k < n; ++k) dst[k] = (float)(h[k] / div);
+    return MCM_OK;
+  };
+  for (int i = 0; i < nI; ++i) {
+    const int64_t iL = items[i].iL;
+    mcm_mc_output& o = outs[i];
+    double normfactor = (double)items[i].nPhotons / T.spectrum[(size_t)iL];
+    if (T.base.beamType == 2 && (T.base.boundaryType == 0 || T.base.boundaryType == 2)) normfactor /= kKillRange * kKillRange;
+    if (T.calcNFR && o.NFR && items[i].nPhotons) {
+      if (!pl->nfrOut.p) MCM_HIP(hipMalloc(&pl->nfrOut.p, (size_t)T.L * sizeof(float)));
+      const int tpb = 256;
+      const int64_t nb = (T.L + tpb - 1) / tpb;
+      hipLaunchKernelGGL(mc_normalize_nfr, dim3((unsigned)nb), dim3(tpb), 0, s, dNFR.as<double>() + (size_t)i * T.L, pl->M.as<uint8_t>(),
+                         pl->muaNorm.as<double>() + (size_t)iL * T.nM, T.V * normfactor, pl->nfrOut.as<float>(), T.L);
+      MCM_HIP(hipGetLastError());
+      MCM_HIP(hipMemcpyAsync(o.NFR, pl->nfrOut.p, (size_t)T.L * sizeof(float), hipMemcpyDeviceToHost, s));
+      MCM_HIP(hipStreamSynchronize(s));
+    }
+    if (T.base.useLC) {
+      const int64_t LLC = (int64_t)T.lcRes0 * T.lcRes0;
+      const double div = LLC > 1 ? (double)T.base.lcFSorNA * (double)T.base.lcFSorNA / LLC * normfactor : normfactor;
+      if ((rc = slice(dImg, (int64_t)i * T.nImage, T.nImage, o.image, div))) return rc;
+    }
+    if (T.nFF && (rc = slice(dFF, (int64_t)i * T.nFF, T.nFF, o.farField, normfactor))) return rc;
+    float* ni[6] = {o.NI_xpos, o.NI_xneg, o.NI_ypos, o.NI_yneg, o.NI_zpos, o.NI_zneg};
+    const double area[6] = {T.dA[0], T.dA[0], T.dA[1], T.dA[1], T.dA[2], T.dA[2]};
+    for (int f = 0; f < 6; ++f)
+      if (T.nNI[f] && (rc = slice(dNI[f], (int64_t)i * T.nNI[f], T.nNI[f], ni[f], area[f] * normfactor))) return rc;
+    o.nPhotons = (double)items[i].nPhotons;
+    o.nPhotonsCollected = (double)coll[i];
+    o.nThreads = (double)pl->blocksBatch * pl->threads;
+    o.simulationTime = minutes;
+    o.examplePaths = nullptr;
+    o.examplePathsElems = 0;
+  }
   return MCM_OK;
 }
 

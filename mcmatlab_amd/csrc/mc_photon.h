@@ -84,6 +84,7 @@ struct McParams {
   // deposition criteria (counters count only media in [minIdx, maxIdx])
   uint32_t minS, maxS, minRefr, maxRefr, minRefl, maxRefl, minI, maxI, minIdx, maxIdx;
   int onlyCollected, evalAtEnd, criteriaActive;
+  int needTime;                // track the time of flight
   // outputs (FP64 tallies)
   double* NFR;
   double* image;
@@ -94,6 +95,7 @@ struct McParams {
 
 struct Photon {
   float i[3], u[3], D[3];
+  float Dfull[3];  // d[a] / |u[a]|: voxel crossing length per axis for the current direction
   int j;
   float mua, mus, g, ri;
   int cdf;
@@ -146,6 +148,12 @@ MCM_FN void rotate_about(const float* r, const float* a, float th, float* o) {
 
 MCM_FN float boundary_distance(float i, float u, float d) {
   return u != 0.f ? (floorf(i) + (u > 0.f ? 1.f : 0.f) - i) * d / u : INFINITY;
+}
+
+// Per-axis crossing lengths of the current direction (after every change
+// of direction).
+MCM_FN void set_crossing_lengths(Photon& P, const McParams& p) {
+  for (int a = 0; a < 3; ++a) P.Dfull[a] = p.d[a] / fabsf(P.u[a]);
 }
 
 MCM_FN bool inside_box(const McParams& p, const float* i) {
@@ -386,10 +394,78 @@ MCM_FN bool launch(Photon& P, const McParams& p, const Medium* media, PacketRng&
     }
   } while (!P.alive && ++attempts < kLaunchAttempts);
   for (int k = 0; k < 3; ++k) P.D[k] = boundary_distance(P.i[k], P.u[k], p.d[k]);
+  set_crossing_lengths(P, p);
   P.insideVolume = inside_box(p, P.i);
   P.stepLeft = -logf(R.next());
   P.scatterings = P.refractions = P.reflections = P.transitions = 0;
   return attempts < kLaunchAttempts;
+}
+
+// A launched packet as handed from the launch kernel to the transport
+// kernel: position and refractive index, direction and time, distances to
+// the next voxel boundaries and the remaining optical depth, and where the
+// packet's random stream stands (block counter, words left in the current
+// block, the unused words) plus flags.  The transport continues the stream
+// exactly where launchPhoton left it, so the trajectory is the same as with
+// an inline launch.  A launch draws at most a few words per attempt and makes
+// at most 10^6 attempts, so the block counter stays below 2^24.
+struct LaunchRecord {
+  float4 a, b, c;
+  uint4 d;
+};
+
+MCM_FN void pack_launch(const Photon& P, const PacketRng& R, bool ok, LaunchRecord& L) {
+  L.a = make_float4(P.i[0], P.i[1], P.i[2], P.ri);
+  L.b = make_float4(P.u[0], P.u[1], P.u[2], P.time);
+  L.c = make_float4(P.D[0], P.D[1], P.D[2], P.stepLeft);
+  const uint32_t flags = (R.block & 0xFFFFFFu) | ((uint32_t)R.have << 24) | (P.insideVolume ? 1u << 28 : 0u) | (ok ? 1u << 29 : 0u) |
+                         (P.alive ? 1u << 30 : 0u);
+  L.d = make_uint4(flags, R.b0, R.b1, R.b2);
+}
+
+// Photon state after launchPhoton from a record (R must have been started on
+// the same packet).  Returns whether the launch succeeded.
+MCM_FN bool unpack_launch(const LaunchRecord& L, Photon& P, PacketRng& R) {
+  P.i[0] = L.a.x;
+  P.i[1] = L.a.y;
+  P.i[2] = L.a.z;
+  P.ri = L.a.w;
+  P.u[0] = L.b.x;
+  P.u[1] = L.b.y;
+  P.u[2] = L.b.z;
+  P.time = L.b.w;
+  P.D[0] = L.c.x;
+  P.D[1] = L.c.y;
+  P.D[2] = L.c.z;
+  P.stepLeft = L.c.w;
+  const uint32_t f = L.d.x;
+  R.block = f & 0xFFFFFFu;
+  R.have = (int)((f >> 24) & 7u);
+  R.b0 = L.d.y;
+  R.b1 = L.d.z;
+  R.b2 = L.d.w;
+  P.insideVolume = (f >> 28) & 1u;
+  P.alive = (f >> 30) & 1u;
+  P.sameVoxel = false;
+  P.weight = 1.f;
+  P.kec = 0;
+  P.substeps = 0;
+  P.scatterings = P.refractions = P.reflections = P.transitions = 0;
+  return (f >> 29) & 1u;
+}
+
+// A packet that left the cuboid or died on a boundary, for the tally kernel
+// (light collector image, far field, edge fluxes): position after
+// checkEscape, direction, weight, time, refractive index, whether it
+// escaped / adds to the edge fluxes, and its work item in a batched launch.
+struct EscapeRecord {
+  float4 a, b, c;
+};
+
+MCM_FN void pack_escape(const Photon& P, bool escaped, bool edgeFlux, int item, EscapeRecord& E) {
+  E.a = make_float4(P.i[0], P.i[1], P.i[2], P.weight);
+  E.b = make_float4(P.u[0], P.u[1], P.u[2], P.time);
+  E.c = make_float4(P.ri, (escaped ? 1.f : 0.f) + (edgeFlux ? 2.f : 0.f), (float)item, 0.f);
 }
 
 // getNewVoxelProperties (MCmatlablib.c:894-904); the refractive index is only
@@ -414,13 +490,13 @@ MCM_FN MoveResult move(Photon& P, const McParams& p, const Medium* media) {
   const float q = P.stepLeft / P.mus;
   const float s = fminf(q, fminf(P.D[0], fminf(P.D[1], P.D[2])));
   P.stepLeft = s == q ? 0.f : P.stepLeft - s * P.mus;
-  P.time += s * P.ri / kC;
+  if (p.needTime) P.time += s * P.ri / kC;
 #pragma unroll
   for (int a = 0; a < 3; ++a) {
     const float fi = floorf(P.i[a]);
     if (s == P.D[a]) {
       P.i[a] = P.u[a] > 0.f ? fi + 1.f : fi - kEps * (fabsf(fi) + 1.f);
-      P.D[a] = p.d[a] / fabsf(P.u[a]);
+      P.D[a] = P.Dfull[a];
       P.sameVoxel = false;
     } else {
       float ni = P.i[a] + s * P.u[a] * p.inv_d[a];
@@ -505,6 +581,7 @@ MCM_FN void fresnel(Photon& P, const McParams& p, const Medium* media, int jNew,
   P.refractions += reflect ? 0u : counted;
   P.transitions += reflect ? 0u : counted;
   for (int a = 0; a < 3; ++a) P.D[a] = boundary_distance(P.i[a], P.u[a], p.d[a]);
+  set_crossing_lengths(P, p);
 }
 
 // xyztoXYZ (MCmatlablib.c:239-246) with precomputed sin/cos
@@ -669,6 +746,7 @@ MCM_FN void scatter(Photon& P, const McParams& p, const float* cdfs, int mCur, P
     P.u[2] = ct * (P.u[2] >= 0.f ? 1.f : -1.f);
   }
   for (int a = 0; a < 3; ++a) P.D[a] = boundary_distance(P.i[a], P.u[a], p.d[a]);
+  set_crossing_lengths(P, p);
   P.stepLeft = -logf(R.next());
   if (p.criteriaActive && in_count_range(mCur, p)) ++P.scatterings;
 }

@@ -284,6 +284,9 @@ inline std::string build_mc_tables(const mcm_mc_input* in, McTables& T) {
                              p.maxRefl == 0xFFFFFFFFu && p.minI == 0 && p.maxI == 0xFFFFFFFFu && !p.onlyCollected;
   if (unrestrictive) p.evalAtEnd = 0;
   p.criteriaActive = !unrestrictive;
+  // the time of flight is read only by time-binned light collectors and the
+  // boundary type 3 age limit
+  p.needTime = (p.useLC && p.lcRes1 > 1) || p.boundaryType == 3;
   T.nExamplePaths = in->nExamplePaths;
   return "";
 }

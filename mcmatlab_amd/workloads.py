@@ -12,6 +12,9 @@ Geometry (all sizes in cm, voxel centres as in geometry.m:130-138):
 - layered_tissue_mc(n=200, matched=False, beam="tophat"): the 200^3 case with
   Fresnel refraction/reflection, a 0.03 cm top-hat focal-plane profile and a
   Gaussian angular profile of 5 degrees 1/e^2 half-angle -- 1e8 packets.
+- layered_tissue_spectral_mc(n=150, nL=64): the matched pencil-beam case with
+  64 wavelengths and per-wavelength packet budgets round(10^U(5,7)) (rng 1),
+  for batched launches.
 - layered_tissue_heat(n=200): the heat run on the same grid: every medium
   kappa 0.0037 W/(cm K), VHC 3.7 J/(cm^3 K); blood Arrhenius E 422.5 kJ/mol,
   A 7.6e66 /s; 1 W for 5 ms then 10 ms off as 667 + 1333 steps; 37 C start;
@@ -68,6 +71,38 @@ def layered_tissue_mc(n=100, matched=True, beam="pencil", nPhotons=1e6, boundary
     else:
         raise ValueError(beam)
     return m
+
+
+SPECTRAL_A = [1e-8, 20.0, 0.5, 230.0]     # mua at 500 nm [1/cm]
+SPECTRAL_B = [1e-8, 200.0, 200.0, 470.0]  # mus at 500 nm [1/cm]
+
+
+def spectral_budgets(nL=64, seed=1):
+    """Packets per wavelength of the spectral workload: round(10^U(5, 7))
+    drawn with numpy.random.default_rng(seed)."""
+    rng = np.random.default_rng(seed)
+    return [int(round(10 ** u)) for u in rng.uniform(5, 7, nL)]
+
+
+def layered_tissue_spectral_mc(n=150, nL=64, side=SIDE):
+    """The layered skin of layered_tissue_mc (matched, pencil beam) on n^3
+    with nL wavelengths linspace(450, 950): mua(l) = a exp(-(l - 500)/150),
+    mus(l) = b (l/500)^-1.3, g = 0.9 (air kept at 1e-8).  Returns (model,
+    budgets)."""
+    m = layered_tissue_mc(n=n, matched=True, beam="pencil", side=side)
+    lam = np.linspace(450.0, 950.0, nL)
+    a = np.asarray(SPECTRAL_A)[:, None]
+    b = np.asarray(SPECTRAL_B)[:, None]
+    mua = a * np.exp(-(lam[None, :] - 500.0) / 150.0)
+    mus = b * (lam[None, :] / 500.0) ** -1.3
+    mua[0, :] = 1e-8
+    mus[0, :] = 1e-8
+    g = np.tile(np.asarray(G_ANISO)[:, None], (1, nL))
+    m.MC.mediaProperties = media_table(mua, mus, g, np.ones((4, nL)))
+    m.MC.spectrum = np.ones(nL)
+    budgets = spectral_budgets(nL)
+    m.MC.nPhotonsRequested = float(sum(budgets))
+    return m, budgets
 
 
 def layered_tissue_heat(n=200, side=SIDE, power=1.0, NFR=None, steps_on=667, steps_off=1333,

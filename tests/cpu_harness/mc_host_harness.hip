@@ -20,6 +20,7 @@ using namespace mcm;
 namespace {
 
 struct HostSink {
+  static constexpr bool kDeferEscapes = false;
   const McParams* p;
   uint64_t collectedN = 0, runawayN = 0;
   void deposit(int j, double v) { p->NFR[j] += v; }
@@ -31,9 +32,67 @@ struct HostSink {
   void runaway() { ++runawayN; }
 };
 
+// 1: launch through LaunchRecord and tally escapes through EscapeRecord, as
+// the GPU kernels do when no deposition criteria are active
+int g_mode = 0;
+
+struct HostDeferSink : HostSink {
+  static constexpr bool kDeferEscapes = true;
+  std::vector<EscapeRecord>* rec;
+  void escape(const Photon& P, bool escaped, bool edgeFlux, uint32_t) {
+    EscapeRecord E;
+    pack_escape(P, escaped, edgeFlux, 0, E);
+    rec->push_back(E);
+  }
+};
+
+template <bool FR, bool CR>
+void run_packets_records(const McParams& p, uint64_t n, uint32_t k0, uint32_t k1, uint32_t stream, uint64_t* launched,
+                         uint64_t* collected, double* stats) {
+  PacketEngine<FR, CR> E;
+  E.init(k0, k1, stream);
+  std::vector<EscapeRecord> rec;
+  HostDeferSink sink;
+  sink.p = &p;
+  sink.rec = &rec;
+  for (uint64_t k = 0; k < n; ++k) {
+    PacketRng R;
+    R.key0 = k0;
+    R.key1 = k1;
+    R.stream = stream;
+    R.start(k);
+    Photon P0;
+    const bool ok = launch(P0, p, p.media, R);
+    LaunchRecord L;
+    pack_launch(P0, R, ok, L);
+    E.replay = false;
+    if (!E.start_from(k, L, p, p.media, p.cdfIdx)) break;
+    ++*launched;
+    while (E.state != ST_FETCH) {
+      if (E.state == ST_MOVE) {
+        stats[0] += 1;
+        E.do_move(p, p.media, p.cdfIdx, sink);
+      } else if (E.state == ST_SCATTER) {
+        stats[1] += 1;
+        E.do_scatter(p, p.media, p.cdfIdx, sink);
+      } else {
+        stats[2] += 1;
+        E.do_fresnel(p, p.media, p.cdfIdx, sink);
+      }
+    }
+  }
+  HostSink tally{&p};
+  for (const EscapeRecord& r : rec) tally_escape(r, p, tally);
+  *collected += tally.collectedN;
+}
+
 template <bool FR, bool CR>
 void run_packets(const McParams& p, uint64_t n, uint32_t k0, uint32_t k1, uint32_t stream, uint64_t* launched, uint64_t* collected,
                  double* stats) {
+  if (!CR && g_mode == 1) {
+    run_packets_records<FR, CR>(p, n, k0, k1, stream, launched, collected, stats);
+    return;
+  }
   PacketEngine<FR, CR> E;
   E.init(k0, k1, stream);
   HostSink sink{&p};
@@ -58,6 +117,8 @@ void run_packets(const McParams& p, uint64_t n, uint32_t k0, uint32_t k1, uint32
 }
 
 }  // namespace
+
+extern "C" void mc_host_set_mode(int mode) { g_mode = mode; }
 
 extern "C" int mc_host_run(const mcm_mc_input* in, mcm_mc_output* out, uint64_t seed, double* stats /* [3] */) {
   McTables T;

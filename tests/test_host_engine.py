@@ -22,15 +22,38 @@ pytestmark = pytest.mark.skipif(not os.path.exists(HARNESS), reason="host harnes
 _lib = None
 
 
-def host_run(m, seed):
+def host_run(m, seed, records=False):
+    """records: launch through LaunchRecord and tally escapes through
+    EscapeRecord (the GPU's launch / transport / tally kernel split)."""
     global _lib
     if _lib is None:
         _lib = C.CDLL(HARNESS)
         _lib.mc_host_run.argtypes = [C.POINTER(abi.mcm_mc_input), C.POINTER(abi.mcm_mc_output), C.c_uint64, C.c_void_p]
+        _lib.mc_host_set_mode.argtypes = [C.c_int]
     inp = abi.MCInputs(m, seed=seed)
     out = abi.MCOutputs(inp)
-    assert _lib.mc_host_run(C.byref(inp.struct), C.byref(out.struct), seed, None) == 0
+    _lib.mc_host_set_mode(1 if records else 0)
+    try:
+        assert _lib.mc_host_run(C.byref(inp.struct), C.byref(out.struct), seed, None) == 0
+    finally:
+        _lib.mc_host_set_mode(0)
     return out.result()
+
+
+@pytest.mark.parametrize("name", ["pencil_bt0", "pencil_bt2", "fresnel_bt1", "fresnel_bt3", "radial_tophat_gauss", "planewave_bt2",
+                                  "lens_timebins", "fiber", "farfield", "source3d", "xy_custom_angular", "isotropic_line"])
+def test_launch_and_escape_records_reproduce_inline_path(name):
+    """The launch record carries the packet and its random stream exactly, and
+    the deferred escape tallies add the same weights in the same order, so
+    both paths give bit-identical outputs."""
+    m = models.cases()[name]()
+    m.MC.nPhotonsRequested = 3000
+    a = host_run(m, 77)
+    b = host_run(m, 77, records=True)
+    assert a["nPhotons"] == b["nPhotons"] and a["nPhotonsCollected"] == b["nPhotonsCollected"]
+    for k, v in a.items():
+        if isinstance(v, np.ndarray) and v.dtype.kind == "f":
+            assert np.array_equal(v, b[k], equal_nan=True), k
 
 
 @pytest.mark.parametrize("name", ["pencil_bt1", "fresnel_bt1", "fresnel_bt3", "radial_tophat_gauss", "planewave_bt2", "lens_timebins",

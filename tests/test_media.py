@@ -43,3 +43,20 @@ def test_sensor_weights_trilinear_corner():
     np.testing.assert_allclose(w[0], [0.5, 0.0, 0.5], atol=1e-12)
     assert li[1] == 0 + 3 * 5 + 0
     np.testing.assert_allclose(w[1], [0.0, 1.0, 0.0])
+
+
+def test_spectral_budget_fixture_matches_generator():
+    import json
+    import os
+
+    from mcmatlab_amd.workloads import layered_tissue_spectral_mc, spectral_budgets
+
+    fx = json.load(open(os.path.join(os.path.dirname(__file__), "golden", "spectral64_budgets.json")))
+    assert fx["budgets"] == spectral_budgets(64)
+    assert 1e5 <= min(fx["budgets"]) and max(fx["budgets"]) <= 1e7
+    m, b = layered_tissue_spectral_mc(n=10, nL=64)
+    mua = m.MC.mediaProperties["mua"]
+    assert mua.shape == (4, 64)
+    lam = np.linspace(450, 950, 64)
+    assert np.allclose(mua[1], 20 * np.exp(-(lam - 500) / 150))
+    assert np.allclose(m.MC.mediaProperties["mus"][3], 470 * (lam / 500) ** -1.3)
