@@ -20,6 +20,7 @@ line (see the keys below), including:
 
 Usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--packets N]
        python bench.py --copy-probe BYTES   (plain copy kernel for counter scaling)
+       python bench.py --workload spectral64   (batched vs plain, 64 wavelengths)
 """
 from __future__ import annotations
 
@@ -156,6 +157,48 @@ def bench_heat(hip, steps_on=667, steps_off=1333):
             "algorithmic_GBs": L * (steps_on * 41 + steps_off * 33) / dt / 1e9}
 
 
+def bench_spectral(seed=7):
+    """The 64-wavelength spectral workload (150^3, matched, pencil beam,
+    per-wavelength budgets round(10^U(5,7))): one batched persistent launch
+    (mcm_mc_batch_run) against 64 plain launches on the same plan and
+    against 64 whole single-wavelength calls (mcm_mc_run each).  Wall clock
+    includes normalisation and the download of every wavelength's outputs."""
+    import numpy as np
+
+    from mcmatlab_amd import api
+    from mcmatlab_amd.workloads import layered_tissue_spectral_mc
+
+    model, budgets = layered_tissue_spectral_mc(n=150, nL=64)
+    total = sum(budgets)
+    plan = api.MCPlan(model)
+    items = [(iL, n, seed, iL) for iL, n in enumerate(budgets)]
+    plan.batch([(0, 100000, seed + 1, 0)])  # warm-up
+    t = time.perf_counter()
+    got = plan.batch(items)
+    t_batch = time.perf_counter() - t
+    t = time.perf_counter()
+    for iL, n in enumerate(budgets):
+        plan.simulate(n, seed=seed, iL=iL, firstPhoton=0)
+        plan.normalize(iL)
+    t_plain = time.perf_counter() - t
+    plan.close()
+    t = time.perf_counter()
+    for iL, n in enumerate(budgets):
+        m1 = model.copy()
+        mp = model.MC.mediaProperties
+        m1.MC.mediaProperties = {k: np.asarray(v)[:, iL:iL + 1] for k, v in mp.items()}
+        m1.MC.spectrum = np.ones(1)
+        m1.MC.nPhotonsRequested = float(n)
+        api.MCmatlab(m1, 1, seed=seed)
+    t_calls = time.perf_counter() - t
+    absorbed = [float(np.sum(np.asarray(g["NFR"], np.float64)[..., 0] * np.asarray(model.MC.mediaProperties["mua"])[:, iL][model.MC.M - 1])
+                      * model.G.dx ** 3) for iL, g in enumerate(got)]
+    return {"metric": "photons/s, 64-wavelength batched launch (150^3 layered skin, pencil beam)", "value": total / t_batch,
+            "unit": "photons/s", "packets": total, "batched_s": t_batch, "plain_launches_s": t_plain, "plain_calls_s": t_calls,
+            "speedup_vs_plain_launches": t_plain / t_batch, "speedup_vs_plain_calls": t_calls / t_batch,
+            "absorbed_fraction_first_last": [absorbed[0], absorbed[-1]]}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -164,6 +207,9 @@ def main():
     ap.add_argument("--packets", type=float, default=1e8)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-heat", action="store_true")
+    ap.add_argument("--workload", default="layered_tissue_200_fresnel_tophat",
+                    choices=["layered_tissue_200_fresnel_tophat", "spectral64"],
+                    help="spectral64: batched vs plain launches of the 64-wavelength workload (prints its own line)")
     ap.add_argument("--copy-probe", type=int, default=0, metavar="BYTES",
                     help="only run device-to-device copies of BYTES (counter calibration) and exit")
     args = ap.parse_args()
@@ -171,6 +217,9 @@ def main():
         print("bench.py measures one GPU only: this build has no multi-GPU path (run with --gpus 1)", file=sys.stderr)
         sys.exit(2)
 
+    if args.workload == "spectral64":
+        print(json.dumps(bench_spectral()), flush=True)
+        return
     if args.copy_probe:
         Hip().copy_probe(args.copy_probe)
         print(json.dumps({"copy_probe_bytes": args.copy_probe, "repeats": 3}), flush=True)

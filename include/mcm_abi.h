@@ -221,6 +221,27 @@ typedef struct mcm_mc_item {
  * MCM_ERR_UNSUPPORTED; nItems * nM must not exceed 2048. */
 int mcm_mc_batch_run(mcm_mc_plan *plan, const mcm_mc_item *items, int64_t nItems, mcm_mc_output *outs);
 
+/* Differentiable absorbed power (no reference counterpart).  J = power
+ * absorbed inside a box of voxels per watt of source power (spectrum[iL]
+ * scales it like the NFR normalisation, MCmatlablib.c:1195-1200), its
+ * gradient with respect to a per-voxel absorption coefficient map, and its
+ * derivative with respect to a common scale s of every medium's mus
+ * (mus -> s mus; for one shared mus, dJ/dmus = dJ_dlogmus / mus).  The map
+ * overrides the media table's mua voxel by voxel (NULL: the table's). */
+typedef struct mcm_mc_grad_input {
+  const float *muaMap;   /* [nx,ny,nz] 1/cm, or NULL */
+  int64_t box[6];        /* voxels x0 <= x < x1, y0 <= y < y1, z0 <= z < z1 (0-based) */
+  int64_t iL;            /* wavelength */
+  uint64_t nPhotons;
+  uint64_t seed;         /* Philox key; the stream is iL, so equal seeds give common random numbers */
+} mcm_mc_grad_input;
+typedef struct mcm_mc_grad_output {
+  double J, J_se;                  /* estimate and its standard error */
+  double *dJ_dmua;                 /* caller-owned [nx,ny,nz] or NULL: dJ/dmua per voxel [W cm / W] */
+  double dJ_dlogmus, dJ_dlogmus_se;
+} mcm_mc_grad_output;
+int mcm_mc_grad_run(mcm_mc_plan *plan, const mcm_mc_grad_input *in, mcm_mc_grad_output *out);
+
 /* ---- heat ---- */
 int mcm_heat_run(const mcm_heat_input *in, mcm_heat_output *out);
 

@@ -256,6 +256,16 @@ def mc_output_sizes(shape, nL, boundaryType, calcNFR, farFieldRes, useLightColle
     return out
 
 
+class mcm_mc_grad_input(C.Structure):
+    _fields_ = [("muaMap", C.POINTER(C.c_float)), ("box", C.c_int64 * 6), ("iL", C.c_int64), ("nPhotons", C.c_uint64),
+                ("seed", C.c_uint64)]
+
+
+class mcm_mc_grad_output(C.Structure):
+    _fields_ = [("J", C.c_double), ("J_se", C.c_double), ("dJ_dmua", C.POINTER(C.c_double)), ("dJ_dlogmus", C.c_double),
+                ("dJ_dlogmus_se", C.c_double)]
+
+
 class mcm_mc_item(C.Structure):
     _fields_ = [("iL", C.c_int64), ("nPhotons", C.c_uint64), ("seed", C.c_uint64), ("stream", C.c_uint32),
                 ("reserved", C.c_uint32)]

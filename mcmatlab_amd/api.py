@@ -136,6 +136,32 @@ class MCPlan:
             res.append(o.result())
         return res
 
+    def absorbed_power(self, box, nPhotons, seed=0, muaMap=None, iL=0, grad=True):
+        """mcm_mc_grad_run: absorbed power in the voxel box ((x0, x1), (y0,
+        y1), (z0, z1), 0-based, half-open) per watt, its gradient with
+        respect to the per-voxel absorption map (muaMap [nx,ny,nz] 1/cm, or
+        None for the media table), and d/d log(mus scale).  Returns a dict
+        J, J_se, dJ_dmua (Fortran-ordered [nx,ny,nz] or None), dJ_dlogmus,
+        dJ_dlogmus_se."""
+        shape = self._inp.shape
+        gi = abi.mcm_mc_grad_input()
+        keep = None
+        if muaMap is not None:
+            keep = np.ascontiguousarray(np.asarray(muaMap, dtype=np.float32).reshape(shape).ravel(order="F"))
+            gi.muaMap = keep.ctypes.data_as(C.POINTER(C.c_float))
+        (x0, x1), (y0, y1), (z0, z1) = box
+        for k, v in enumerate((x0, x1, y0, y1, z0, z1)):
+            gi.box[k] = int(v)
+        gi.iL, gi.nPhotons, gi.seed = int(iL), int(nPhotons), int(seed)
+        go = abi.mcm_mc_grad_output()
+        g = np.zeros(int(np.prod(shape)), dtype=np.float64) if grad else None
+        if g is not None:
+            go.dJ_dmua = g.ctypes.data_as(C.POINTER(C.c_double))
+        check(lib().mcm_mc_grad_run(self.h, C.byref(gi), C.byref(go)))
+        del keep
+        return {"J": go.J, "J_se": go.J_se, "dJ_dmua": None if g is None else g.reshape(shape, order="F"),
+                "dJ_dlogmus": go.dJ_dlogmus, "dJ_dlogmus_se": go.dJ_dlogmus_se}
+
     def close(self):
         if self.h:
             lib().mcm_mc_plan_destroy(self.h)
@@ -146,3 +172,28 @@ class MCPlan:
             self.close()
         except Exception:
             pass
+
+
+def absorbed_power_function():
+    """A torch.autograd.Function over MCPlan.absorbed_power: forward(muaMap)
+    returns J (a 0-d float64 tensor), backward multiplies dJ/dmua by the
+    incoming gradient.  Usage:
+        AP = absorbed_power_function()
+        J = AP.apply(mua_tensor, plan, box, nPhotons, seed)
+        J.backward()   # mua_tensor.grad = dJ/dmua
+    """
+    import torch
+
+    class AbsorbedPower(torch.autograd.Function):
+        @staticmethod
+        def forward(ctx, mua, plan, box, nPhotons, seed):
+            r = plan.absorbed_power(box, nPhotons, seed=seed, muaMap=mua.detach().cpu().numpy())
+            ctx.save_for_backward(torch.from_numpy(np.ascontiguousarray(r["dJ_dmua"])).to(mua.device, mua.dtype))
+            return torch.tensor(r["J"], dtype=torch.float64, device=mua.device)
+
+        @staticmethod
+        def backward(ctx, gJ):
+            (g,) = ctx.saved_tensors
+            return gJ.to(g.dtype) * g, None, None, None, None
+
+    return AbsorbedPower

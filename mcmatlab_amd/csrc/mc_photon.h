@@ -108,6 +108,7 @@ struct Photon {
 
 // Result of the movement half of a sub-step.
 struct MoveResult {
+  float s;       // geometric length of the sub-step [cm]
   float absorb;
   int jNew;
   int mNew;
@@ -519,6 +520,7 @@ MCM_FN MoveResult move(Photon& P, const McParams& p, const Medium* media) {
       if (r.mNew != mOld && media[mOld].ri == P.ri && in_count_range(r.mNew, p)) ++P.transitions;
     }
   }
+  r.s = s;
   r.absorb = -P.weight * expm1f(-P.mua * s);
   P.weight -= r.absorb;
   return r;

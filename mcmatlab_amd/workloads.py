@@ -15,6 +15,8 @@ Geometry (all sizes in cm, voxel centres as in geometry.m:130-138):
 - layered_tissue_spectral_mc(n=150, nL=64): the matched pencil-beam case with
   64 wavelengths and per-wavelength packet budgets round(10^U(5,7)) (rng 1),
   for batched launches.
+- blob_absorber_mc(n=128): a scatterer with a per-voxel absorption map of
+  Gaussian blobs and a 16^3 target box, for absorbed-power gradients.
 - layered_tissue_heat(n=200): the heat run on the same grid: every medium
   kappa 0.0037 W/(cm K), VHC 3.7 J/(cm^3 K); blood Arrhenius E 422.5 kJ/mol,
   A 7.6e66 /s; 1 W for 5 ms then 10 ms off as 667 + 1333 steps; 37 C start;
@@ -103,6 +105,37 @@ def layered_tissue_spectral_mc(n=150, nL=64, side=SIDE):
     budgets = spectral_budgets(nL)
     m.MC.nPhotonsRequested = float(sum(budgets))
     return m, budgets
+
+
+def blob_absorber_mc(n=128, side=SIDE, seed=2, nBlobs=8):
+    """A homogeneous scatterer (mus 100/cm, g 0.85, n 1.37 everywhere, so
+    refraction happens only at the cuboid surface) with a per-voxel mua map
+    1 + sum of 8 Gaussian blobs (peak U[5, 50] /cm, sigma U[4, 12] voxels,
+    centres uniform; numpy.random.default_rng(seed)), a pencil beam along z
+    and the target box voxels [56,72) x [56,72) x [24,40) scaled to n.
+    Returns (model, muaMap float32 [n,n,n], box)."""
+    rng = np.random.default_rng(seed)
+    d = side / n
+    m = Model()
+    m.G = Geometry(dx=d, dy=d, dz=d)
+    mc = m.MC
+    mc.M = np.ones((n, n, n), np.uint8)
+    mc.mediaProperties = media_table([1.0], [100.0], [0.85], [1.37])
+    mc.matchedInterfaces = True
+    mc.interfaceNormals = np.float32(np.nan)
+    mc.boundaryType = 1
+    mc.spectrum = np.array([1.0])
+    mc.LS.sourceType = 0
+    mc.nPhotonsRequested = 1e7
+    X, Y, Z = np.meshgrid(np.arange(n), np.arange(n), np.arange(n), indexing="ij")
+    mua = np.ones((n, n, n))
+    for _ in range(nBlobs):
+        peak, sig = rng.uniform(5, 50), rng.uniform(4, 12) * n / 128
+        c = rng.uniform(0, n, 3)
+        mua += peak * np.exp(-((X - c[0]) ** 2 + (Y - c[1]) ** 2 + (Z - c[2]) ** 2) / (2 * sig ** 2))
+    s = n / 128
+    box = ((int(56 * s), int(72 * s)), (int(56 * s), int(72 * s)), (int(24 * s), int(40 * s)))
+    return m, mua.astype(np.float32), box
 
 
 def layered_tissue_heat(n=200, side=SIDE, power=1.0, NFR=None, steps_on=667, steps_off=1333,
