@@ -253,6 +253,34 @@ int mcm_heat_plan_advance(mcm_heat_plan *plan, int64_t steps, double dt, int32_t
 int mcm_heat_plan_download(mcm_heat_plan *plan, mcm_heat_output *out);
 int mcm_heat_plan_destroy(mcm_heat_plan *plan);
 
+/* Discrete adjoint of the FP64 DG-ADI propagation (no reference
+ * counterpart).  Runs the schedule (nSegments segments of `steps` steps of
+ * size dt with illumination lightsOn; nSegments == 0: in->steps, in->dt,
+ * in->lightsOn) from in->T / in->Omega and returns
+ *   J      = sum over the target voxels of the final damage integral Omega
+ *            (targetMask [nx,ny,nz] non-zero, or NULL = voxels whose medium
+ *            has E != 0),
+ *   dJ_dQ  = dJ / d dTdt_abs per voxel (lights-on steps only) [1 per K/s],
+ *   dJ_dT0 = dJ / d T(initial) per voxel [1/K],
+ * the exact derivatives of the discrete scheme (up to rounding).  States are
+ * checkpointed every checkpointEvery steps (0: 100) and recomputed one
+ * segment at a time on the way back.  Needs precision MCM_HEAT_FP64, scheme
+ * MCM_HEAT_DGADI and a damage input (Omega). */
+typedef struct mcm_heat_segment {
+  int64_t steps;
+  double dt;
+  int32_t lightsOn;
+  int32_t reserved;
+} mcm_heat_segment;
+typedef struct mcm_heat_adjoint_output {
+  double J;
+  double *dJ_dQ;      /* caller-owned [nx,ny,nz] or NULL */
+  double *dJ_dT0;     /* caller-owned [nx,ny,nz] or NULL */
+  double *Omega_f64;  /* caller-owned final damage [nx,ny,nz] or NULL */
+} mcm_heat_adjoint_output;
+int mcm_heat_adjoint_run(const mcm_heat_input *in, const mcm_heat_segment *segments, int64_t nSegments,
+                         const uint8_t *targetMask, int64_t checkpointEvery, mcm_heat_adjoint_output *out);
+
 #ifdef __cplusplus
 }
 #endif

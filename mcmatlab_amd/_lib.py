@@ -30,6 +30,8 @@ SIGNATURES = {
     "mcm_heat_plan_advance": (C.c_int, [C.c_void_p, C.c_int64, C.c_double, C.c_int32]),
     "mcm_heat_plan_download": (C.c_int, [C.c_void_p, C.POINTER(abi.mcm_heat_output)]),
     "mcm_heat_plan_destroy": (C.c_int, [C.c_void_p]),
+    "mcm_heat_adjoint_run": (C.c_int, [C.POINTER(abi.mcm_heat_input), C.POINTER(abi.mcm_heat_segment), C.c_int64,
+                                       C.POINTER(C.c_uint8), C.c_int64, C.POINTER(abi.mcm_heat_adjoint_output)]),
 }
 
 _lib = None

@@ -266,6 +266,15 @@ class mcm_mc_grad_output(C.Structure):
                 ("dJ_dlogmus_se", C.c_double)]
 
 
+class mcm_heat_segment(C.Structure):
+    _fields_ = [("steps", C.c_int64), ("dt", C.c_double), ("lightsOn", C.c_int32), ("reserved", C.c_int32)]
+
+
+class mcm_heat_adjoint_output(C.Structure):
+    _fields_ = [("J", C.c_double), ("dJ_dQ", C.POINTER(C.c_double)), ("dJ_dT0", C.POINTER(C.c_double)),
+                ("Omega_f64", C.POINTER(C.c_double))]
+
+
 class mcm_mc_item(C.Structure):
     _fields_ = [("iL", C.c_int64), ("nPhotons", C.c_uint64), ("seed", C.c_uint64), ("stream", C.c_uint32),
                 ("reserved", C.c_uint32)]

@@ -88,6 +88,39 @@ def finiteElementHeatPropagator(T, Omega, P, precision="fp64", scheme="dgadi", s
 finiteElementHeatPropagator.last_state = (None, None)
 
 
+def heat_adjoint(T, Omega, P, segments=None, targetMask=None, checkpointEvery=0, grads=True, T_f64=None, Omega_f64=None):
+    """mcm_heat_adjoint_run: final damage J summed over the target voxels
+    (targetMask, default: media with E != 0) after the schedule `segments`
+    = [(steps, dt, lightsOn), ...] (default: P's steps, dt, lightsOn) of FP64
+    DG-ADI steps, and its exact discrete gradients with respect to the
+    heating rate dTdt_abs and the initial temperature.  Returns a dict J,
+    dJ_dQ, dJ_dT0 ([nx,ny,nz] Fortran order, or None), Omega (final FP64
+    damage)."""
+    inp = abi.HeatInputs(T, Omega, P, precision=abi.MCM_HEAT_FP64, scheme=abi.MCM_HEAT_DGADI, T_f64=T_f64, Omega_f64=Omega_f64)
+    shape = (int(inp.struct.nx), int(inp.struct.ny), int(inp.struct.nz))
+    L = int(np.prod(shape))
+    segs = segments or []
+    arr = (abi.mcm_heat_segment * max(len(segs), 1))()
+    for k, (steps, dt, on) in enumerate(segs):
+        arr[k].steps, arr[k].dt, arr[k].lightsOn, arr[k].reserved = int(steps), float(dt), int(bool(on)), 0
+    mask = None
+    if targetMask is not None:
+        mask = np.ascontiguousarray(np.asarray(targetMask, dtype=np.uint8).reshape(shape).ravel(order="F"))
+    out = abi.mcm_heat_adjoint_output()
+    om = np.zeros(L)
+    out.Omega_f64 = om.ctypes.data_as(C.POINTER(C.c_double))
+    gq = gt = None
+    if grads:
+        gq, gt = np.zeros(L), np.zeros(L)
+        out.dJ_dQ = gq.ctypes.data_as(C.POINTER(C.c_double))
+        out.dJ_dT0 = gt.ctypes.data_as(C.POINTER(C.c_double))
+    check(lib().mcm_heat_adjoint_run(C.byref(inp.struct), arr, len(segs),
+                                     None if mask is None else mask.ctypes.data_as(C.POINTER(C.c_uint8)),
+                                     int(checkpointEvery), C.byref(out)))
+    f = lambda a: None if a is None else a.reshape(shape, order="F")
+    return {"J": out.J, "dJ_dQ": f(gq), "dJ_dT0": f(gt), "Omega": f(om)}
+
+
 class MCPlan:
     """Device-resident Monte Carlo plan (mcm_mc_plan_*): upload once, launch
     many times.  Used by the benchmark and by sweeps."""

@@ -594,6 +594,152 @@ __global__ void heat_init_keys(unsigned long long* k, int n) {
   if (i < n) k[i] = order_key(-INFINITY);
 }
 
+
+// ---------------------------------------------------------------------------
+// Discrete adjoint of the FP64 DG-ADI step (no reference counterpart).
+//
+// With D the non-frozen voxels and L_d the conduction operators
+// ((L_d u)_i = sum over the d-neighbours n of c[M_i, M_n, d] (u_n - u_i)),
+// one step is
+//   T*   = u + dt D (L_x/2 + L_y + L_z) u + dt lightsOn D Q
+//   T1   = A_x^-1 T*,  T2 = A_y^-1 (T1 - dt/2 D L_y u),  T3 = A_z^-1 (T2 - dt/2 D L_z u)
+// with A_d = I - dt/2 D L_d.  For an adjoint lam of T3 the adjoints are
+//   mu3 = A_z^-T lam,  mu2 = A_y^-T mu3,  mu1 = A_x^-T mu2
+//   lam_u = mu1 + dt (L_x/2 + L_y + L_z)^T D mu1 - dt/2 L_y^T D mu2 - dt/2 L_z^T D mu3
+//   lam_Q = dt lightsOn D mu1
+// and the damage of the step, dt A exp(-E / (R ((T3 + u)/2 + 273.15))),
+// adds dt A exp(.) E / (R theta^2) / 2 to the adjoints of both u and T3.
+
+// adjoint of the step's damage term for the target voxels
+__global__ void heat_adj_damage(HeatArgs<double> a, const double* __restrict__ x0, const double* __restrict__ x1,
+                                const uint8_t* __restrict__ target, double* lam, int64_t L) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= L || !target[i]) return;
+  const int m = a.M[i];
+  const double E = a.E[m];
+  if (E == 0) return;
+  const double th = (x0[i] + x1[i]) / 2 + a.czero;
+  lam[i] += a.dt * exp(a.logA[m] - E / (a.Rgas * th)) * E / (a.Rgas * th * th) / 2;
+}
+
+// A_d^T y = r on every line along axis d: lane per line, Thomas recurrence;
+// `beta` holds the eliminated diagonal.
+template <int AXIS>
+__global__ void __launch_bounds__(kHeatTPB) heat_adj_solve(HeatArgs<double> a, const double* __restrict__ r, double* y, double* beta) {
+  const int nx = a.nx, ny = a.ny, nz = a.nz, nxy = nx * ny, nM = a.nM;
+  const int line = blockIdx.x * blockDim.x + threadIdx.x;
+  int i0, st, n;
+  bool lf;
+  if (AXIS == 0) {
+    if (line >= ny * nz) return;
+    const int yy = line % ny, zz = line / ny;
+    i0 = yy * nx + zz * nxy, st = 1, n = nx;
+    lf = a.sinked && (yy == 0 || yy == ny - 1 || zz == 0 || zz == nz - 1);
+  } else if (AXIS == 1) {
+    if (line >= nx * nz) return;
+    const int xx = line % nx, zz = line / nx;
+    i0 = xx + zz * nxy, st = nx, n = ny;
+    lf = a.sinked && (xx == 0 || xx == nx - 1 || zz == 0 || zz == nz - 1);
+  } else {
+    if (line >= nxy) return;
+    const int xx = line % nx, yy = line / nx;
+    i0 = line, st = nxy, n = nz;
+    lf = a.sinked && (xx == 0 || xx == nx - 1 || yy == 0 || yy == ny - 1);
+  }
+  const double h = a.dt / 2;
+  // row k of A_d: (sub, diag, super) = (ak, bk, ck)
+  auto row = [&](int k, double& ak, double& bk, double& ck) {
+    ak = 0;
+    bk = 1;
+    ck = 0;
+    if (lf || (a.sinked && (k == 0 || k == n - 1))) return;
+    const int i = i0 + k * st, m = a.M[i];
+    if (k > 0) {
+      const double q = h * cc(a, m, a.M[i - st], AXIS);
+      bk += q;
+      ak = -q;
+    }
+    if (k < n - 1) {
+      const double q = h * cc(a, m, a.M[i + st], AXIS);
+      bk += q;
+      ck = -q;
+    }
+  };
+  double aPrev, bPrev, cPrev, ak, bk, ck;
+  row(0, ak, bk, ck);
+  double bet = bk, yy = r[i0];
+  beta[i0] = bet;
+  y[i0] = yy;
+  for (int k = 1; k < n; ++k) {
+    aPrev = ak;
+    bPrev = bk;
+    cPrev = ck;
+    row(k, ak, bk, ck);
+    // A^T row k: sub = c_(k-1), diag = b_k; A^T row k-1 has super = a_k
+    const double w = cPrev / bet;
+    bet = bk - w * ak;
+    const int i = i0 + k * st;
+    yy = r[i] - w * yy;
+    beta[i] = bet;
+    y[i] = yy;
+    (void)aPrev;
+    (void)bPrev;
+  }
+  double xn = y[i0 + (n - 1) * st] / beta[i0 + (n - 1) * st];
+  y[i0 + (n - 1) * st] = xn;
+  double aNext;
+  {
+    double t0, t1;
+    row(n - 1, aNext, t0, t1);
+  }
+  for (int k = n - 2; k >= 0; --k) {
+    const int i = i0 + k * st;
+    xn = (y[i] - aNext * xn) / beta[i];
+    y[i] = xn;
+    double t0, t1;
+    row(k, aNext, t0, t1);
+  }
+}
+
+__device__ inline bool heat_frozen(const HeatArgs<double>& a, int x, int y, int z) { return frozen(a, x, y, z); }
+
+// lam_u of the step (plus the damage adjoint of u) and the source adjoint.
+__global__ void __launch_bounds__(kHeatTPB) heat_adj_combine(HeatArgs<double> a, const double* __restrict__ mu1, const double* __restrict__ mu2,
+                                                            const double* __restrict__ mu3, const double* __restrict__ x0,
+                                                            const double* __restrict__ x1, const uint8_t* __restrict__ target,
+                                                            double* lamOut, double* dQ, int lightsOn) {
+  const int nx = a.nx, ny = a.ny, nz = a.nz, nxy = nx * ny, nM = a.nM;
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= (int64_t)nxy * nz) return;
+  const int x = (int)(i % nx), y = (int)((i / nx) % ny), z = (int)(i / nxy);
+  const int m = a.M[i];
+  const double dt = a.dt;
+  // L_d^T (D w) at i for w = mu1, mu2, mu3
+  auto wv = [&](const double* mu, int xx, int yy, int zz) { return heat_frozen(a, xx, yy, zz) ? 0.0 : mu[xx + yy * nx + (int64_t)zz * nxy]; };
+  double t1[3] = {0, 0, 0}, t2 = 0, t3 = 0;
+  const int xyz[3] = {x, y, z}, ext[3] = {nx, ny, nz};
+  for (int d = 0; d < 3; ++d) {
+    for (int s = -1; s <= 1; s += 2) {
+      const int q = xyz[d] + s;
+      if (q < 0 || q >= ext[d]) continue;
+      const int xx = d == 0 ? q : x, yy = d == 1 ? q : y, zz = d == 2 ? q : z;
+      const int mn = a.M[xx + yy * nx + (int64_t)zz * nxy];
+      const double cin = cc(a, mn, m, d), cout = cc(a, m, mn, d);
+      t1[d] += cin * wv(mu1, xx, yy, zz) - cout * wv(mu1, x, y, z);
+      if (d == 1) t2 += cin * wv(mu2, xx, yy, zz) - cout * wv(mu2, x, y, z);
+      if (d == 2) t3 += cin * wv(mu3, xx, yy, zz) - cout * wv(mu3, x, y, z);
+    }
+  }
+  double lam = mu1[i] + dt * (t1[0] / 2 + t1[1] + t1[2]) - dt / 2 * t2 - dt / 2 * t3;
+  if (target[i] && a.E[m] != 0) {
+    const double E = a.E[m];
+    const double th = (x0[i] + x1[i]) / 2 + a.czero;
+    lam += dt * exp(a.logA[m] - E / (a.Rgas * th)) * E / (a.Rgas * th * th) / 2;
+  }
+  lamOut[i] = lam;
+  if (lightsOn) dQ[i] += dt * wv(mu1, x, y, z);
+}
+
 }  // namespace mcm
 
 using namespace mcm;
@@ -918,6 +1064,99 @@ int mcm_heat_plan_destroy(mcm_heat_plan* P) {
   (void)hipSetDevice(P->device);
   (void)hipStreamSynchronize(P->stream);
   delete P;
+  return MCM_OK;
+}
+
+int mcm_heat_adjoint_run(const mcm_heat_input* in, const mcm_heat_segment* segments, int64_t nSegments, const uint8_t* targetMask,
+                         int64_t checkpointEvery, mcm_heat_adjoint_output* out) {
+  if (!in || !out) return fail(MCM_ERR_INVALID_ARGUMENT, "null argument");
+  if (in->precision != MCM_HEAT_FP64 || in->scheme != MCM_HEAT_DGADI)
+    return fail(MCM_ERR_UNSUPPORTED, "the adjoint is implemented for the FP64 DG-ADI scheme");
+  if (!in->Omega && !in->Omega_f64) return fail(MCM_ERR_INVALID_ARGUMENT, "the objective is the damage integral: Omega is required");
+  if (nSegments < 0 || (nSegments > 0 && !segments)) return fail(MCM_ERR_INVALID_ARGUMENT, "segments missing");
+  std::vector<double> dts;
+  std::vector<int> lights;
+  if (nSegments == 0) {
+    dts.assign((size_t)std::max<int64_t>(in->steps, 0), in->dt);
+    lights.assign(dts.size(), in->lightsOn != 0);
+  }
+  for (int64_t k = 0; k < nSegments; ++k) {
+    if (segments[k].steps < 0) return fail(MCM_ERR_INVALID_ARGUMENT, "segment steps must be non-negative");
+    dts.insert(dts.end(), (size_t)segments[k].steps, segments[k].dt);
+    lights.insert(lights.end(), (size_t)segments[k].steps, segments[k].lightsOn != 0);
+  }
+  const int64_t N = (int64_t)dts.size();
+  const int64_t C = checkpointEvery > 0 ? checkpointEvery : 100;
+  mcm_heat_plan* raw = nullptr;
+  int rc = mcm_heat_plan_create(in, &raw);
+  if (rc) return rc;
+  std::unique_ptr<mcm_heat_plan, int (*)(mcm_heat_plan*)> PP(raw, mcm_heat_plan_destroy);
+  mcm_heat_plan* P = PP.get();
+  const int64_t L = P->L;
+  const size_t bytes = (size_t)L * sizeof(double);
+  hipStream_t s = P->stream;
+  // target voxels: the mask, or every voxel whose medium has E != 0
+  std::vector<uint8_t> mask((size_t)L);
+  for (int64_t i = 0; i < L; ++i) mask[i] = targetMask ? (targetMask[i] != 0) : (in->E[in->M[i]] != 0);
+  const int64_t nChk = std::max<int64_t>(1, (N + C - 1) / C);
+  HeatBuf dMask, chk;
+  if ((rc = halloc(dMask, (size_t)L)) || (rc = halloc(chk, (size_t)nChk * bytes))) return rc;
+  MCM_HIP(hipMemcpyAsync(dMask.p, mask.data(), (size_t)L, hipMemcpyHostToDevice, s));
+  auto chkp = [&](int64_t k) { return (double*)chk.p + (size_t)k * L; };
+  // forward pass with checkpoints every C steps
+  for (int64_t n = 0; n < N; ++n) {
+    if (n % C == 0) MCM_HIP(hipMemcpyAsync(chkp(n / C), P->T1.p, bytes, hipMemcpyDeviceToDevice, s));
+    if ((rc = advance<double>(P, 1, dts[(size_t)n], lights[(size_t)n]))) return rc;
+  }
+  if (N == 0) MCM_HIP(hipMemcpyAsync(chkp(0), P->T1.p, bytes, hipMemcpyDeviceToDevice, s));
+  std::vector<double> om((size_t)L);
+  MCM_HIP(hipMemcpyAsync(om.data(), P->Om.p, bytes, hipMemcpyDeviceToHost, s));
+  MCM_HIP(hipStreamSynchronize(s));
+  double J = 0;
+  for (int64_t i = 0; i < L; ++i)
+    if (mask[i]) J += om[i];
+  out->J = J;
+  if (out->Omega_f64) std::memcpy(out->Omega_f64, om.data(), bytes);
+  if (!out->dJ_dQ && !out->dJ_dT0) return MCM_OK;
+  // reverse sweep, one checkpoint segment at a time
+  HeatBuf seg, lam, lamN, mu1, mu2, mu3, dQ;
+  if ((rc = halloc(seg, (size_t)(C + 1) * bytes)) || (rc = halloc(lam, bytes)) || (rc = halloc(lamN, bytes)) || (rc = halloc(mu1, bytes)) ||
+      (rc = halloc(mu2, bytes)) || (rc = halloc(mu3, bytes)) || (rc = halloc(dQ, bytes)))
+    return rc;
+  MCM_HIP(hipMemsetAsync(lam.p, 0, bytes, s));
+  MCM_HIP(hipMemsetAsync(dQ.p, 0, bytes, s));
+  auto segp = [&](int64_t k) { return (double*)seg.p + (size_t)k * L; };
+  const int tpb = kHeatTPB;
+  const unsigned gv = (unsigned)((L + tpb - 1) / tpb);
+  const unsigned gx = (unsigned)(((int64_t)P->ny * P->nz + tpb - 1) / tpb);
+  const unsigned gy = (unsigned)(((int64_t)P->nx * P->nz + tpb - 1) / tpb);
+  const unsigned gz = (unsigned)(((int64_t)P->nx * P->ny + tpb - 1) / tpb);
+  for (int64_t k = (N + C - 1) / C - 1; k >= 0; --k) {
+    const int64_t s0 = k * C, e0 = std::min(N, s0 + C);
+    MCM_HIP(hipMemcpyAsync(P->T1.p, chkp(k), bytes, hipMemcpyDeviceToDevice, s));
+    MCM_HIP(hipMemcpyAsync(segp(0), chkp(k), bytes, hipMemcpyDeviceToDevice, s));
+    for (int64_t n = s0; n < e0; ++n) {
+      if ((rc = advance<double>(P, 1, dts[(size_t)n], lights[(size_t)n]))) return rc;
+      MCM_HIP(hipMemcpyAsync(segp(n - s0 + 1), P->T1.p, bytes, hipMemcpyDeviceToDevice, s));
+    }
+    for (int64_t n = e0 - 1; n >= s0; --n) {
+      HeatArgs<double> a = make_args<double>(P, dts[(size_t)n], lights[(size_t)n]);
+      const double* x0 = segp(n - s0);
+      const double* x1 = segp(n - s0 + 1);
+      double* B = (double*)P->B.p;
+      hipLaunchKernelGGL(heat_adj_damage, dim3(gv), dim3(tpb), 0, s, a, x0, x1, (const uint8_t*)dMask.p, (double*)lam.p, L);
+      hipLaunchKernelGGL(heat_adj_solve<2>, dim3(gz), dim3(tpb), 0, s, a, (const double*)lam.p, (double*)mu3.p, B);
+      hipLaunchKernelGGL(heat_adj_solve<1>, dim3(gy), dim3(tpb), 0, s, a, (const double*)mu3.p, (double*)mu2.p, B);
+      hipLaunchKernelGGL(heat_adj_solve<0>, dim3(gx), dim3(tpb), 0, s, a, (const double*)mu2.p, (double*)mu1.p, B);
+      hipLaunchKernelGGL(heat_adj_combine, dim3(gv), dim3(tpb), 0, s, a, (const double*)mu1.p, (const double*)mu2.p, (const double*)mu3.p, x0,
+                         x1, (const uint8_t*)dMask.p, (double*)lamN.p, (double*)dQ.p, lights[(size_t)n]);
+      MCM_HIP(hipGetLastError());
+      std::swap(lam.p, lamN.p);
+    }
+  }
+  if (out->dJ_dQ) MCM_HIP(hipMemcpyAsync(out->dJ_dQ, dQ.p, bytes, hipMemcpyDeviceToHost, s));
+  if (out->dJ_dT0) MCM_HIP(hipMemcpyAsync(out->dJ_dT0, lam.p, bytes, hipMemcpyDeviceToHost, s));
+  MCM_HIP(hipStreamSynchronize(s));
   return MCM_OK;
 }
 

@@ -38,7 +38,7 @@ def test_library_exports_every_header_symbol():
 
 
 STRUCTS = ["mcm_beam_distr", "mcm_mc_input", "mcm_mc_output", "mcm_mc_sizes", "mcm_mc_item", "mcm_mc_grad_input",
-           "mcm_mc_grad_output", "mcm_heat_input", "mcm_heat_output"]
+           "mcm_mc_grad_output", "mcm_heat_input", "mcm_heat_output", "mcm_heat_segment", "mcm_heat_adjoint_output"]
 
 
 def test_ctypes_layout_matches_c(tmp_path):
