@@ -20,7 +20,7 @@ line (see the keys below), including:
 
 Usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--packets N]
        python bench.py --copy-probe BYTES   (plain copy kernel for counter scaling)
-       python bench.py --workload spectral64   (batched vs plain, 64 wavelengths)
+       python bench.py --workload spectral64|heat200|grad128|adjoint128   (secondary lines)
 """
 from __future__ import annotations
 
@@ -95,6 +95,21 @@ class Hip:
             self._ok(self.h.hipDeviceSynchronize(), "hipDeviceSynchronize")
         else:
             self._ok(self.h.hipStreamSynchronize(s), "hipStreamSynchronize")
+
+
+def measured_traffic(packets_per_launch):
+    """HBM bytes per transport launch from the committed counter summary
+    (rocprofv3 FETCH_SIZE / WRITE_SIZE runs scaled by a copy probe), or None
+    when it was measured for another launch size."""
+    path = os.path.join(ROOT, "profiles", "mc_transport_traffic.json")
+    try:
+        with open(path) as f:
+            d = json.load(f)
+    except OSError:
+        return None
+    if d.get("workload") != WORKLOAD or int(d.get("packets_per_launch", 0)) != packets_per_launch:
+        return None
+    return float(d["traffic_bytes_per_launch"])
 
 
 def roofline_counts():
@@ -199,6 +214,56 @@ def bench_spectral(seed=7):
             "absorbed_fraction_first_last": [absorbed[0], absorbed[-1]]}
 
 
+def bench_heat_line():
+    """The heat propagator alone: 667 lights-on + 1333 lights-off FP64 DG-ADI
+    steps on the 200^3 grid, against the compulsory traffic of 41 / 33 bytes
+    per voxel-step."""
+    r = bench_heat(Hip())
+    L = 200 ** 3
+    alg = L * (667 * 41 + 1333 * 33)
+    gbs = alg / r["seconds"] / 1e9
+    return {"metric": "voxel-steps/s, 200^3 heat (FP64 DG-ADI, 667 on + 1333 off)", "value": r["voxel_steps_per_s"],
+            "unit": "voxel-steps/s", "ms_per_step": r["seconds"] / r["steps"] * 1e3, "dtype": "f64",
+            "roofline": {"bound": "hbm", "achieved": gbs, "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": gbs / HBM_PEAK_GBS}}
+
+
+def bench_grad(n=128, packets=10_000_000, seed=3):
+    """Absorbed power in the 16^3 box of the 128^3 blob absorber and its
+    per-voxel mua gradient (two passes per packet)."""
+    from mcmatlab_amd import api
+    from mcmatlab_amd.workloads import blob_absorber_mc
+
+    m, mua, box = blob_absorber_mc(n=n)
+    plan = api.MCPlan(m)
+    plan.absorbed_power(box, 100000, seed=seed + 1, muaMap=mua)  # warm-up
+    t = time.perf_counter()
+    r = plan.absorbed_power(box, packets, seed=seed, muaMap=mua)
+    dt = time.perf_counter() - t
+    plan.close()
+    return {"metric": "packets/s, absorbed power + dJ/dmua (128^3 blob absorber, 16^3 box)", "value": packets / dt,
+            "unit": "packets/s", "seconds": dt, "J": r["J"], "J_rel_se": r["J_se"] / r["J"],
+            "dJ_dlogmus": r["dJ_dlogmus"], "dJ_dlogmus_rel_se": abs(r["dJ_dlogmus_se"] / r["dJ_dlogmus"])}
+
+
+def bench_adjoint(n=128, steps=3000, every=100):
+    """Final blood damage and its gradient w.r.t. the heating rate: 3000 FP64
+    DG-ADI steps on 128^3 (heat parameters of the 200^3 workload),
+    checkpoints every 100 steps."""
+    from mcmatlab_amd import api
+    from mcmatlab_amd.workloads import layered_tissue_heat
+
+    T0, Om0, Pon, _ = layered_tissue_heat(n=n, steps_on=steps, steps_off=1)
+    t = time.perf_counter()
+    r = api.heat_adjoint(T0, Om0, Pon, segments=[(steps, 5e-3 / 667, True)], checkpointEvery=every, grads=False)
+    t_fwd = time.perf_counter() - t
+    t = time.perf_counter()
+    r = api.heat_adjoint(T0, Om0, Pon, segments=[(steps, 5e-3 / 667, True)], checkpointEvery=every)
+    dt = time.perf_counter() - t
+    return {"metric": "seconds, 3000-step heat adjoint (128^3, FP64 DG-ADI, checkpoints every 100)", "value": dt, "unit": "s",
+            "higher_is_better": False, "forward_only_s": t_fwd, "adjoint_over_forward": dt / t_fwd, "J": r["J"],
+            "dJ_dQ_max": float(abs(r["dJ_dQ"]).max())}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -208,8 +273,10 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-heat", action="store_true")
     ap.add_argument("--workload", default="layered_tissue_200_fresnel_tophat",
-                    choices=["layered_tissue_200_fresnel_tophat", "spectral64"],
-                    help="spectral64: batched vs plain launches of the 64-wavelength workload (prints its own line)")
+                    choices=["layered_tissue_200_fresnel_tophat", "spectral64", "heat200", "grad128", "adjoint128"],
+                    help="secondary workloads (each prints its own line): spectral64 batched vs plain launches of the "
+                         "64-wavelength workload; heat200 the heat propagator alone; grad128 absorbed-power gradients; "
+                         "adjoint128 the heat adjoint")
     ap.add_argument("--copy-probe", type=int, default=0, metavar="BYTES",
                     help="only run device-to-device copies of BYTES (counter calibration) and exit")
     args = ap.parse_args()
@@ -217,8 +284,9 @@ def main():
         print("bench.py measures one GPU only: this build has no multi-GPU path (run with --gpus 1)", file=sys.stderr)
         sys.exit(2)
 
-    if args.workload == "spectral64":
-        print(json.dumps(bench_spectral()), flush=True)
+    secondary = {"spectral64": bench_spectral, "heat200": bench_heat_line, "grad128": bench_grad, "adjoint128": bench_adjoint}
+    if args.workload in secondary:
+        print(json.dumps(secondary[args.workload]()), flush=True)
         return
     if args.copy_probe:
         Hip().copy_probe(args.copy_probe)
@@ -288,7 +356,7 @@ def main():
                    "beam": "top-hat r=0.03 cm, Gaussian 5 deg divergence", "boundaryType": 1,
                    "absorbed_fraction": absorbed},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": achieved / HBM_PEAK_GBS, "traffic": None,
+                     "frac": achieved / HBM_PEAK_GBS, "traffic": measured_traffic(n_per),
                      "kernel": "mc_transport", "launch_ms": avg_ms, "packets_per_launch": n_per,
                      "bytes_per_packet": rc["bytes_per_packet"]},
     }
