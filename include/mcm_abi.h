@@ -199,6 +199,16 @@ int mcm_mc_plan_tallies(mcm_mc_plan *plan, const double **d_deposit, uint64_t *n
  * [7] lanes per launch. */
 int mcm_mc_plan_counters(mcm_mc_plan *plan, double counters[8]);
 int mcm_mc_plan_destroy(mcm_mc_plan *plan);
+/* Launch tuning (no reference counterpart).  The fluence accumulates into 1,
+ * 2, 4 or 8 FP64 copies, one per group of the GPU's XCDs, summed before any
+ * read: more copies relieve atomic contention on hot voxels (narrow beams),
+ * fewer keep the working set small (broad beams).  Times one launch of
+ * nPackets packets (0: 2^21) of wavelength iL per copy count (within a 2 GiB
+ * budget), keeps the fewest copies within 10 % of the fastest trial, then
+ * clears the plan's tallies and counters.
+ * ms[4] (optional) receives the trial times for 1, 2, 4, 8 copies (0 =
+ * skipped).  mcm_mc_run calls it for runs of 2e7 packets or more. */
+int mcm_mc_plan_autotune(mcm_mc_plan *plan, int64_t iL, uint64_t nPackets, double ms[4]);
 
 /* Batched launch (no reference counterpart: the reference runs one
  * wavelength after another, MCmatlab.c:619-747).  Item i asks for nPhotons

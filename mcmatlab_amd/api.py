@@ -152,6 +152,14 @@ class MCPlan:
         keys = ("launched", "collected", "substeps", "scatterings", "interfaces", "runaway", "aborted", "lanes")
         return dict(zip(keys, buf[:]))
 
+    def autotune(self, iL=0, nPackets=0):
+        """mcm_mc_plan_autotune: pick the fastest number of XCD-private
+        fluence copies for this model; clears the tallies.  Returns the trial
+        times in ms for 1, 2, 4, 8 copies (0 = skipped)."""
+        buf = (C.c_double * 4)()
+        check(lib().mcm_mc_plan_autotune(self.h, int(iL), int(nPackets), buf))
+        return dict(zip((1, 2, 4, 8), buf[:]))
+
     def batch(self, items):
         """mcm_mc_batch_run: items = [(iL, nPhotons, seed, stream), ...] run
         in one sequence of persistent launches; returns one output dict per

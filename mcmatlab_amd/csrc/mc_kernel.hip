@@ -108,8 +108,16 @@ struct DeviceSink {
   int accJ;
   unsigned long long nCollected;
   unsigned nRunaway;
+  // the accelerator complex die (XCD) the wavefront runs on; each XCD has
+  // its own L2, so XCD-private fluence copies keep hot voxels from being
+  // contended across XCDs
+  __device__ static int xcd() {
+    unsigned r;
+    asm volatile("s_getreg_b32 %0, hwreg(HW_REG_XCC_ID)" : "=s"(r));
+    return (int)(r & 7u);
+  }
   __device__ explicit DeviceSink(const McParams& p)
-      : NFR(p.NFR), image(p.image), FF(p.FF), ni0(p.NI[0]), ni1(p.NI[1]), ni2(p.NI[2]), ni3(p.NI[3]), ni4(p.NI[4]),
+      : NFR(p.NFR + (size_t)(xcd() % p.nfrReplicas) * p.L), image(p.image), FF(p.FF), ni0(p.NI[0]), ni1(p.NI[1]), ni2(p.NI[2]), ni3(p.NI[3]), ni4(p.NI[4]),
         ni5(p.NI[5]), acc(0.0), accJ(-1), nCollected(0), nRunaway(0) {}
   __device__ void deposit(int j, double v) {
     if (j == accJ) {
@@ -358,6 +366,18 @@ __global__ void mc_normalize_nfr(double* dep, const uint8_t* M, const double* mu
   dep[j] = 0.0;
 }
 
+// Sum the fluence copies 1..R-1 into copy 0 and clear them.
+__global__ void mc_fold_replicas(double* dep, int64_t L, int R) {
+  const int64_t j = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (j >= L) return;
+  double s = dep[j];
+  for (int r = 1; r < R; ++r) {
+    s += dep[(size_t)r * L + j];
+    dep[(size_t)r * L + j] = 0.0;
+  }
+  dep[j] = s;
+}
+
 __global__ void mc_fill_normal_ri(float4* nri, const uint8_t* M, const Medium* media, int64_t L) {
   const int64_t j = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (j >= L) return;
@@ -490,6 +510,35 @@ int select_wavelength(mcm_mc_plan* pl, int64_t iL) {
 }  // namespace mcm
 
 namespace {
+
+// Fold the fluence copies into copy 0 (on the plan's stream).
+int fold_replicas(mcm_mc_plan* pl) {
+  const int R = pl->prm.nfrReplicas;
+  if (R <= 1 || !pl->T.calcNFR) return MCM_OK;
+  const int tpb = 256;
+  const int64_t nb = (pl->T.L + tpb - 1) / tpb;
+  MCM_HIP(hipDeviceSynchronize());  // launches on other streams write the copies
+  hipLaunchKernelGGL(mc_fold_replicas, dim3((unsigned)nb), dim3(tpb), 0, pl->stream, pl->NFR.as<double>(), pl->T.L, R);
+  MCM_HIP(hipGetLastError());
+  return MCM_OK;
+}
+
+// Number of fluence copies: folds the current tallies, then reallocates.
+int set_replicas(mcm_mc_plan* pl, int R) {
+  if (R == pl->prm.nfrReplicas || !pl->T.calcNFR) return MCM_OK;
+  int rc = fold_replicas(pl);
+  if (rc) return rc;
+  MCM_HIP(hipStreamSynchronize(pl->stream));
+  DevBuf nb;
+  MCM_HIP(hipMalloc(&nb.p, (size_t)R * pl->T.L * sizeof(double)));
+  MCM_HIP(hipMemsetAsync(nb.p, 0, (size_t)R * pl->T.L * sizeof(double), pl->stream));
+  MCM_HIP(hipMemcpyAsync(nb.p, pl->NFR.p, (size_t)pl->T.L * sizeof(double), hipMemcpyDeviceToDevice, pl->stream));
+  MCM_HIP(hipStreamSynchronize(pl->stream));
+  std::swap(pl->NFR.p, nb.p);
+  pl->prm.nfrReplicas = R;
+  pl->prm.NFR = pl->NFR.as<double>();
+  return MCM_OK;
+}
 
 int ensure_records(mcm_mc_plan* pl, uint64_t n) {
   if (n <= pl->recCap) return MCM_OK;
@@ -682,6 +731,9 @@ int mcm_mc_plan_tallies(mcm_mc_plan* pl, const double** d_deposit, uint64_t* nPh
   MCM_HIP(hipStreamSynchronize(pl->stream));
   McCounters c;
   MCM_HIP(hipMemcpy(&c, pl->counters.p, sizeof c, hipMemcpyDeviceToHost));
+  int rc = fold_replicas(pl);
+  if (rc) return rc;
+  MCM_HIP(hipStreamSynchronize(pl->stream));
   if (d_deposit) *d_deposit = pl->NFR.as<double>();
   if (nPhotons) *nPhotons = c.launched;
   return MCM_OK;
@@ -700,6 +752,8 @@ int mcm_mc_plan_normalize(mcm_mc_plan* pl, int64_t iL, mcm_mc_output* out) {
   double normfactor = (double)c.launched / pl->power;
   if (pl->sourceDistribution) normfactor /= pl->power;
   else if (T.base.beamType == 2 && (T.base.boundaryType == 0 || T.base.boundaryType == 2)) normfactor /= kKillRange * kKillRange;
+  int rc0 = fold_replicas(pl);
+  if (rc0) return rc0;
   if (T.calcNFR && out->NFR) {
     if (!pl->nfrOut.p) MCM_HIP(hipMalloc(&pl->nfrOut.p, (size_t)T.L * sizeof(float)));
     const int tpb = 256;
@@ -735,6 +789,56 @@ int mcm_mc_plan_normalize(mcm_mc_plan* pl, int64_t iL, mcm_mc_output* out) {
   MCM_HIP(hipStreamSynchronize(s));
   pl->lastLaunched = c.launched;
   pl->lastCollected = c.collected;
+  return MCM_OK;
+}
+
+int mcm_mc_plan_autotune(mcm_mc_plan* pl, int64_t iL, uint64_t nPackets, double* ms) {
+  if (!pl) return fail(MCM_ERR_INVALID_ARGUMENT, "null plan");
+  if (iL < 0 || iL >= pl->T.nL) return fail(MCM_ERR_INVALID_ARGUMENT, "wavelength index out of range");
+  MCM_HIP(hipSetDevice(pl->device));
+  const McTables& T = pl->T;
+  if (nPackets == 0) nPackets = 1ull << 21;
+  const int cand[4] = {1, 2, 4, 8};
+  double t[4] = {0, 0, 0, 0};
+  int best = pl->prm.nfrReplicas;
+  double bestMs = INFINITY;
+  hipEvent_t e0, e1;
+  MCM_HIP(hipEventCreate(&e0));
+  MCM_HIP(hipEventCreate(&e1));
+  int rc = MCM_OK;
+  for (int c = 0; c < 4 && rc == MCM_OK; ++c) {
+    const int R = cand[c];
+    if (R > 1 && (!T.calcNFR || (double)R * T.L * 8 > 2147483648.0)) continue;
+    if ((rc = set_replicas(pl, R))) break;
+    if ((rc = mcm_mc_plan_simulate(pl, iL, std::min<uint64_t>(nPackets, 1u << 16), 0x7E57ull, 0, nullptr))) break;  // warm-up
+    (void)hipEventRecord(e0, pl->stream);
+    if ((rc = mcm_mc_plan_simulate(pl, iL, nPackets, 0x7E57ull, 1ull << 40, nullptr))) break;
+    (void)hipEventRecord(e1, pl->stream);
+    (void)hipEventSynchronize(e1);
+    float f = 0.f;
+    (void)hipEventElapsedTime(&f, e0, e1);
+    t[c] = f;
+    bestMs = std::min<double>(bestMs, f);
+  }
+  // the fewest copies within 10 % of the fastest: more copies only pay off
+  // clearly (contended narrow beams); otherwise keep the working set small
+  for (int c = 3; c >= 0; --c)
+    if (t[c] > 0 && t[c] <= 1.10 * bestMs) best = cand[c];
+  (void)hipEventDestroy(e0);
+  (void)hipEventDestroy(e1);
+  if (rc) return rc;
+  if ((rc = set_replicas(pl, best))) return rc;
+  // discard the trial tallies
+  hipStream_t s = pl->stream;
+  MCM_HIP(hipDeviceSynchronize());
+  if (T.calcNFR) MCM_HIP(hipMemsetAsync(pl->NFR.p, 0, (size_t)pl->prm.nfrReplicas * T.L * sizeof(double), s));
+  MCM_HIP(hipMemsetAsync(pl->image.p, 0, (size_t)std::max<int64_t>(T.nImage, 1) * sizeof(double), s));
+  MCM_HIP(hipMemsetAsync(pl->FF.p, 0, (size_t)std::max<int64_t>(T.nFF, 1) * sizeof(double), s));
+  for (int f = 0; f < 6; ++f) MCM_HIP(hipMemsetAsync(pl->NI[f].p, 0, (size_t)std::max<int64_t>(T.nNI[f], 1) * sizeof(double), s));
+  MCM_HIP(hipMemsetAsync(pl->counters.p, 0, sizeof(McCounters), s));
+  MCM_HIP(hipStreamSynchronize(s));
+  if (ms)
+    for (int c = 0; c < 4; ++c) ms[c] = t[c];
   return MCM_OK;
 }
 
@@ -897,6 +1001,10 @@ int mcm_mc_run(const mcm_mc_input* in, mcm_mc_output* out) {
   std::unique_ptr<mcm_mc_plan, int (*)(mcm_mc_plan*)> pl(raw, mcm_mc_plan_destroy);
   McTables& T = pl->T;
   const unsigned long long nReq = T.timed ? ~0ull : (unsigned long long)T.nPhotonsRequested;
+  // large runs first pick the fastest number of fluence copies for this model
+  if ((T.timed || nReq >= 20000000ull) && !pl->prm.criteriaActive && T.calcNFR) {
+    if ((rc = mcm_mc_plan_autotune(pl.get(), 0, 0, nullptr))) return rc;
+  }
   double cumPhotons = 0, cumCollected = 0, cumMinutes = 0;
   bool aborted = false, interrupted = false;
   const int nL = T.nL;

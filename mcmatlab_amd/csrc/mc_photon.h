@@ -91,6 +91,7 @@ struct McParams {
   double* FF;
   double* NI[6];  // xpos, xneg, ypos, yneg, zpos, zneg
   int calcNFR;
+  int nfrReplicas;  // copies of NFR (1, 2, 4 or 8), one per group of XCDs; folded before use
 };
 
 struct Photon {

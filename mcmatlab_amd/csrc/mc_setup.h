@@ -160,6 +160,7 @@ inline std::string build_mc_tables(const mcm_mc_input* in, McTables& T) {
   p.nM = T.nM;
   p.L = T.L;
   p.calcNFR = T.calcNFR;
+  p.nfrReplicas = 1;
   T.V = in->dx * in->dy * in->dz;
   T.dA[0] = in->dy * in->dz;
   T.dA[1] = in->dx * in->dz;

@@ -113,3 +113,24 @@ def test_gpu_invalid_input_is_an_error(api):
     m.MC.M[0, 0, 0] = 9  # medium index beyond the media table
     with pytest.raises(MCMError):
         api.MCmatlab(m, 1, seed=1)
+
+
+@pytest.mark.parametrize("name", ["pencil_bt1", "fresnel_bt1"])
+def test_gpu_fluence_copies_keep_results(api, name):
+    """The autotuner's XCD-private fluence copies are summed before any read,
+    so a tuned plan gives the untuned plan's outputs (up to FP64 summation
+    order), and tuning leaves no trial deposits behind."""
+    m = models.cases()[name]()
+    a = api.MCPlan(m)
+    b = api.MCPlan(m)
+    try:
+        ms = b.autotune(nPackets=20000)
+        assert ms[1] > 0 and ms[8] > 0
+        for plan in (a, b):
+            plan.simulate(30000, seed=5, firstPhoton=0)
+        ra, rb = a.normalize(), b.normalize()
+    finally:
+        a.close()
+        b.close()
+    for k in ("NFR", "NI_zneg", "NI_zpos"):
+        np.testing.assert_allclose(rb[k], ra[k], rtol=1e-5, atol=1e-6 * float(np.max(np.abs(ra[k]))))
