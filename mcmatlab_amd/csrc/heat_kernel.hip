@@ -505,6 +505,107 @@ __global__ void __launch_bounds__(kHeatTPB) heat_sweep_x_wave(HeatArgs<double> a
   }
 }
 
+// substeps 2a + 2b (KIND Y) and 3a + 3b with the Arrhenius update and the
+// per-medium maxima (KIND Z) in FP64 with one wavefront per line: lane l
+// holds voxels l*S .. l*S+S-1 of the line.  Consecutive wavefronts take
+// consecutive x, so the strided loads of a workgroup share cache lines.
+// The right-hand side is T1 of the previous substep minus dt/2 L_d T(step
+// start) along the line, in the expression order of the sequential kernel.
+template <int S, int KIND>
+__global__ void __launch_bounds__(kHeatTPB) heat_sweep_yz_wave(HeatArgs<double> a) {
+  __shared__ double sc[kLdsCoef];
+  __shared__ double sLogA[256], sE[256];
+  __shared__ unsigned long long sMax[256];
+  const int nc = a.nM * a.nM * 3;
+  if (nc <= kLdsCoef)
+    for (int k = threadIdx.x; k < nc; k += blockDim.x) sc[k] = a.c[k];
+  if (KIND == SWEEP_Z)
+    for (int k = threadIdx.x; k < a.nM; k += blockDim.x) {
+      sLogA[k] = a.logA[k];
+      sE[k] = a.E[k];
+      sMax[k] = order_key(-INFINITY);
+    }
+  __syncthreads();
+  const double* __restrict__ cT = nc <= kLdsCoef ? sc : a.c;
+  const int lane = __lane_id();
+  const int line = blockIdx.x * (kHeatTPB / 64) + (threadIdx.x >> 6);
+  const int nx = a.nx, ny = a.ny, nz = a.nz, nxy = nx * ny, nM = a.nM;
+  const int nLines = KIND == SWEEP_Y ? nx * nz : nxy;
+  const bool sk = a.sinked;
+  const double dt = a.dt;
+  if (line < nLines) {
+    const int x = line % nx, o = line / nx;  // o = z (Y) or y (Z)
+    const int i0 = KIND == SWEEP_Y ? x + o * nxy : x + o * nx;
+    const int st = KIND == SWEEP_Y ? nx : nxy;
+    const int n = KIND == SWEEP_Y ? ny : nz;
+    const int d = KIND == SWEEP_Y ? 1 : 2;
+    const bool lf = sk && (x == 0 || x == nx - 1 || o == 0 || o == (KIND == SWEEP_Y ? nz : ny) - 1);
+    const double* __restrict__ T1 = a.T1;
+    const uint8_t* __restrict__ Mg = a.M;
+    double Tn[S], R0[S];
+    int Mv[S];
+#pragma unroll
+    for (int s = 0; s < S; ++s) {
+      const int k = lane * S + s;
+      const int i = i0 + k * st;
+      Tn[s] = k < n ? T1[i] : 0.0;
+      R0[s] = k < n ? a.T2[i] : 0.0;
+      Mv[s] = k < n ? (int)Mg[i] : 0;
+    }
+    const double Tl = lane_shfl(Tn[S - 1], lane - 1), Tr = lane_shfl(Tn[0], lane + 1);
+    const int Ml = __shfl(Mv[S - 1], lane > 0 ? lane - 1 : 0, 64), Mr = __shfl(Mv[0], lane < 63 ? lane + 1 : 63, 64);
+    double ca[S], cb[S], cc_[S], cd[S], xs[S];
+#pragma unroll
+    for (int s = 0; s < S; ++s) {
+      const int k = lane * S + s;
+      ca[s] = 0.0;
+      cb[s] = 1.0;
+      cc_[s] = 0.0;
+      cd[s] = 0.0;
+      if (k >= n) continue;
+      const int m = Mv[s];
+      const double TL = s > 0 ? Tn[s - 1] : Tl, TR = s < S - 1 ? Tn[s + 1] : Tr;
+      const int ML = s > 0 ? Mv[s - 1] : Ml, MR = s < S - 1 ? Mv[s + 1] : Mr;
+      const bool fz = lf || (sk && (k == 0 || k == n - 1));
+      double t = R0[s];
+      if (!fz) {
+        double dTdt = 0;
+        if (k != 0) dTdt -= (TL - Tn[s]) * coef(cT, nM, m, ML, d) / 2;
+        if (k != n - 1) dTdt -= (TR - Tn[s]) * coef(cT, nM, m, MR, d) / 2;
+        t = t + dt * dTdt;
+        if (k < n - 1) {
+          const double q = dt / 2 * coef(cT, nM, m, MR, d);
+          cb[s] += q;
+          cc_[s] = -q;
+        }
+        if (k > 0) {
+          const double q = dt / 2 * coef(cT, nM, m, ML, d);
+          cb[s] += q;
+          ca[s] = -q;
+        }
+      }
+      cd[s] = t;
+    }
+    wave_tridiag<S>(ca, cb, cc_, cd, xs);
+    double* __restrict__ T2 = a.T2;
+    MaxRun run;
+#pragma unroll
+    for (int s = 0; s < S; ++s) {
+      const int k = lane * S + s;
+      if (k >= n) continue;
+      const int i = i0 + k * st;
+      T2[i] = xs[s];
+      if (KIND == SWEEP_Z) {
+        const int m = Mv[s];
+        if (a.calcDamage && sE[m] != 0) a.Om[i] += dt * exp(sLogA[m] - sE[m] / (a.Rgas * ((xs[s] + Tn[s]) / 2 + a.czero)));
+        run.add(m, xs[s], sMax);
+      }
+    }
+    if (KIND == SWEEP_Z) run.flush(sMax);
+  }
+  if (KIND == SWEEP_Z) fold_lds_max(sMax, a.maxKey, a.nM);
+}
+
 // rows per lane of the wavefront x-solve for a line of n voxels (0: use the
 // sequential lane-per-line kernel)
 inline int wave_rows(int n) { return n <= 64 ? 1 : n <= 128 ? 2 : n <= 256 ? 4 : n <= 512 ? 8 : 0; }
@@ -848,6 +949,32 @@ void launch_sweep_x(mcm_heat_plan* P, const HeatArgs<RT>& a, unsigned gx) {
   hipLaunchKernelGGL(heat_sweep_x<RT>, dim3(gx), dim3(kHeatTPB), 0, P->stream, a);
 }
 
+// y and z sweeps: the wavefront solver for FP64 lines of up to 512 voxels,
+// else the sequential lane-per-line kernels.
+template <int KIND>
+void launch_yz_wave(mcm_heat_plan* P, const HeatArgs<double>& a, int n, int64_t lines) {
+  const unsigned gw = (unsigned)((lines + kHeatTPB / 64 - 1) / (kHeatTPB / 64));
+  switch (wave_rows(n)) {
+    case 1: hipLaunchKernelGGL((heat_sweep_yz_wave<1, KIND>), dim3(gw), dim3(kHeatTPB), 0, P->stream, a); break;
+    case 2: hipLaunchKernelGGL((heat_sweep_yz_wave<2, KIND>), dim3(gw), dim3(kHeatTPB), 0, P->stream, a); break;
+    case 4: hipLaunchKernelGGL((heat_sweep_yz_wave<4, KIND>), dim3(gw), dim3(kHeatTPB), 0, P->stream, a); break;
+    default: hipLaunchKernelGGL((heat_sweep_yz_wave<8, KIND>), dim3(gw), dim3(kHeatTPB), 0, P->stream, a); break;
+  }
+}
+
+template <typename RT>
+void launch_sweep_yz(mcm_heat_plan* P, const HeatArgs<RT>& a, unsigned gy, unsigned gz) {
+  if constexpr (std::is_same<RT, double>::value) {
+    if (wave_rows(P->ny)) launch_yz_wave<SWEEP_Y>(P, a, P->ny, (int64_t)P->nx * P->nz);
+    else hipLaunchKernelGGL(heat_sweep_y<RT>, dim3(gy), dim3(kHeatTPB), 0, P->stream, a);
+    if (wave_rows(P->nz)) launch_yz_wave<SWEEP_Z>(P, a, P->nz, (int64_t)P->nx * P->ny);
+    else hipLaunchKernelGGL(heat_sweep_z<RT>, dim3(gz), dim3(kHeatTPB), 0, P->stream, a);
+    return;
+  }
+  hipLaunchKernelGGL(heat_sweep_y<RT>, dim3(gy), dim3(kHeatTPB), 0, P->stream, a);
+  hipLaunchKernelGGL(heat_sweep_z<RT>, dim3(gz), dim3(kHeatTPB), 0, P->stream, a);
+}
+
 template <typename RT>
 int advance(mcm_heat_plan* P, int64_t steps, double dt, int lightsOn) {
   int rc = ensure_sensor_capacity(P, P->stepsDone + steps + 1);
@@ -867,8 +994,7 @@ int advance(mcm_heat_plan* P, int64_t steps, double dt, int lightsOn) {
       hipLaunchKernelGGL(heat_explicit<RT>, dim3(ge), dim3(tpb), 0, P->stream, a);
     } else {
       launch_sweep_x<RT>(P, a, gx);
-      hipLaunchKernelGGL(heat_sweep_y<RT>, dim3(gy), dim3(tpb), 0, P->stream, a);
-      hipLaunchKernelGGL(heat_sweep_z<RT>, dim3(gz), dim3(tpb), 0, P->stream, a);
+      launch_sweep_yz<RT>(P, a, gy, gz);
     }
     MCM_HIP(hipGetLastError());
     std::swap(P->T1.p, P->T2.p);

@@ -126,9 +126,7 @@ def test_gpu_heat_invalid_medium_is_an_error(api):
 
 
 def line_case(nx, ny, nz, bt, seed=0, steps=6):
-    """Random 4-medium block with an nx-voxel x extent (the FP64 x-solve runs
-    one wavefront per x-line for nx <= 512 and the lane-per-line recurrence
-    above that)."""
+    """Random 4-medium block of nx x ny x nz voxels."""
     rng = np.random.default_rng(seed)
     d = 0.1 / 64
     M = rng.integers(0, 4, (nx, ny, nz)).astype(np.uint8)
@@ -143,10 +141,15 @@ def line_case(nx, ny, nz, bt, seed=0, steps=6):
 
 
 @pytest.mark.parametrize("bt", [0, 1])
-@pytest.mark.parametrize("nx", [1, 64, 65, 129, 300, 512, 513])
-def test_gpu_heat_x_line_lengths(api, nx, bt):
-    print(f"[gpu-heat] x-line {nx} bt {bt}", flush=True)
-    T0, Om0, P = line_case(nx, 3, 4, bt, seed=nx)
+@pytest.mark.parametrize("axis", [0, 1, 2])
+@pytest.mark.parametrize("n", [1, 64, 65, 129, 300, 512, 513])
+def test_gpu_heat_line_lengths(api, n, axis, bt):
+    """FP64 lines of up to 512 voxels are solved by one wavefront each (rows
+    per lane 1, 2, 4, 8), longer ones by the lane-per-line recurrence."""
+    print(f"[gpu-heat] axis {axis} line {n} bt {bt}", flush=True)
+    shape = [3, 4, 5]
+    shape[axis] = n
+    T0, Om0, P = line_case(*shape, bt, seed=n + 1000 * axis)
     api.finiteElementHeatPropagator(T0, Om0, P, precision="fp64")
     T64, Om64 = api.finiteElementHeatPropagator.last_state
     ref = OL.oracle_heat(T0, Om0, P, variant=OL.HEAT_FP64_DGADI)
