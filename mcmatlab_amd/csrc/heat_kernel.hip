@@ -3,30 +3,34 @@
 // (+MCmatlab/src/finiteElementHeatPropagator.c:688-821).
 //
 // One Douglas-Gunn ADI step (:767-782) is three line-sweep kernels instead of
-// the reference's six launches:
-//   sweep_x   substeps 1a + 1b: the explicit right-hand side (full 7-point
+// the reference's six loops:
+//   x sweep   substeps 1a + 1b: the explicit right-hand side (full 7-point
 //             stencil, x half-weighted, plus the source) built on the fly and
-//             Thomas-solved along x;
-//   sweep_y   substeps 2a + 2b: the -dt/2 L_y correction and the y solve;
-//   sweep_z   substeps 3a + 3b: the -dt/2 L_z correction, the z solve, the
+//             solved along x;
+//   y sweep   substeps 2a + 2b: the -dt/2 L_y correction and the y solve;
+//   z sweep   substeps 3a + 3b: the -dt/2 L_z correction, the z solve, the
 //             Arrhenius damage update and the per-medium maximum temperature.
-// Each lane owns one grid line.  For the y and z sweeps lanes run along x, so
-// every load and store of a wavefront is one contiguous 64-voxel segment.
-// The forward-eliminated diagonal of each line goes to a scratch array in the
-// same layout (it is re-read by the same lane right after, from cache).
-// The explicit (forward Euler) scheme is one kernel per step: one lane per
-// (x, y) column walking z, keeping the z neighbours in registers.
+// FP64 (default), lines of up to 512 voxels: each line is solved by one
+// wavefront (partition method + parallel cyclic reduction across the 64
+// lanes); the x sweep reads its lines directly (contiguous), the y and z
+// sweeps stage tiles of lines through LDS.  FP32 (the reference's arithmetic)
+// and longer lines: one lane per line running the Thomas recurrence over
+// blocks of loaded elements.
+// The explicit (forward Euler) scheme is one kernel per step.
 //
-// Arithmetic is the reference's expression for expression (built without
-// FMA contraction), in FP64 (default) or in FP32 like the reference.
+// Arithmetic follows the reference's expressions (built without FMA
+// contraction); in FP32 it is the reference's operation for operation.
 // Per-medium maxima use an order-preserving integer key of the FP64 value:
-// each lane keeps a running maximum over its current run of one medium along
-// its line and folds it into LDS when the medium changes; each workgroup
-// folds LDS into the global keys at the end.
+// each lane keeps a running maximum over its current run of one medium and
+// folds it into LDS when the medium changes; each workgroup folds LDS into
+// the global keys at the end.
+// The discrete adjoint of the FP64 DG-ADI step (transposed line solves and
+// stencils) follows the forward kernels.
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
 #include <cmath>
+#include <cstdlib>
 #include <cstring>
 #include <memory>
 #include <type_traits>
@@ -337,6 +341,16 @@ __global__ void __launch_bounds__(kHeatTPB) heat_sweep_z(HeatArgs<RT> a) {
 // Same solution as the sequential Thomas recurrence up to rounding.
 __device__ inline double lane_shfl(double v, int src) { return __shfl(v, src < 0 ? 0 : (src > 63 ? 63 : src), 64); }
 
+// a / b for the positive, well-scaled pivots of the solver: hardware
+// reciprocal estimate refined by two Newton steps (a few ulp), instead of the
+// correctly rounded division sequence.
+__device__ inline double pdiv(double a, double b) {
+  double r = __builtin_amdgcn_rcp(b);
+  r = fma(fma(-b, r, 1.0), r, r);
+  r = fma(fma(-b, r, 1.0), r, r);
+  return a * r;
+}
+
 template <int S>
 __device__ inline void wave_tridiag(const double (&a)[S], const double (&b)[S], const double (&c)[S], const double (&d)[S],
                                     double (&x)[S]) {
@@ -352,7 +366,7 @@ __device__ inline void wave_tridiag(const double (&a)[S], const double (&b)[S], 
     de[1] = d[1];
 #pragma unroll
     for (int s = 2; s <= S - 2; ++s) {
-      const double r = a[s] / be[s - 1];
+      const double r = pdiv(a[s], be[s - 1]);
       al[s] = -r * al[s - 1];
       be[s] = b[s] - r * ga[s - 1];
       ga[s] = c[s];
@@ -361,16 +375,16 @@ __device__ inline void wave_tridiag(const double (&a)[S], const double (&b)[S], 
     // backward elimination, last unknown kept symbolic
 #pragma unroll
     for (int s = S - 3; s >= 1; --s) {
-      const double r = ga[s] / be[s + 1];
+      const double r = pdiv(ga[s], be[s + 1]);
       al[s] -= r * al[s + 1];
       ga[s] = -r * ga[s + 1];
       de[s] -= r * de[s + 1];
     }
-    const double r1 = c[0] / be[1];
+    const double r1 = pdiv(c[0], be[1]);
     B1 = b[0] - r1 * al[1];
     C1 = -r1 * ga[1];
     D1 = d[0] - r1 * de[1];
-    const double r2 = a[S - 1] / be[S - 2];
+    const double r2 = pdiv(a[S - 1], be[S - 2]);
     A2 = -r2 * al[S - 2];
     B2 = b[S - 1] - r2 * ga[S - 2];
     D2 = d[S - 1] - r2 * de[S - 2];
@@ -389,10 +403,11 @@ __device__ inline void wave_tridiag(const double (&a)[S], const double (&b)[S], 
     const double sA = lane_shfl(A1, lane + 1), sB = lane_shfl(B1, lane + 1);
     const double sC = lane_shfl(C1, lane + 1), sD = lane_shfl(D1, lane + 1);
     const double nA = hasNext ? sA : 0.0, nB = hasNext ? sB : 1.0, nC = hasNext ? sC : 0.0, nD = hasNext ? sD : 0.0;
-    pa = -A2 * A1 / B1;
-    pb = B2 - A2 * C1 / B1 - C2 * nA / nB;
-    pc = -C2 * nC / nB;
-    pd = D2 - A2 * D1 / B1 - C2 * nD / nB;
+    const double iB1 = pdiv(1.0, B1), inB = pdiv(1.0, nB);
+    pa = -A2 * A1 * iB1;
+    pb = B2 - A2 * C1 * iB1 - C2 * nA * inB;
+    pc = -C2 * nC * inB;
+    pd = D2 - A2 * D1 * iB1 - C2 * nD * inB;
   }
 #pragma unroll
   for (int h = 1; h < 64; h <<= 1) {
@@ -403,24 +418,24 @@ __device__ inline void wave_tridiag(const double (&a)[S], const double (&b)[S], 
     const double scp = lane_shfl(pc, lane + h), sdp = lane_shfl(pd, lane + h);
     const double am = lo ? sam : 0.0, bm = lo ? sbm : 1.0, cm = lo ? scm : 0.0, dm = lo ? sdm : 0.0;
     const double ap = hi ? sap : 0.0, bp = hi ? sbp : 1.0, cp = hi ? scp : 0.0, dp = hi ? sdp : 0.0;
-    const double k1 = pa / bm, k2 = pc / bp;
+    const double k1 = pdiv(pa, bm), k2 = pdiv(pc, bp);
     pb = pb - k1 * cm - k2 * ap;
     pd = pd - k1 * dm - k2 * dp;
     pa = -k1 * am;
     pc = -k2 * cp;
   }
-  const double U = pd / pb;
+  const double U = pdiv(pd, pb);
   if constexpr (S == 1) {
     x[0] = U;
   } else {
     const double sU = lane_shfl(U, lane - 1);
     const double Up = lane > 0 ? sU : 0.0;
-    const double first = (D1 - A1 * Up - C1 * U) / B1;
+    const double first = pdiv(D1 - A1 * Up - C1 * U, B1);
     x[0] = first;
     x[S - 1] = U;
     if constexpr (S >= 3) {
 #pragma unroll
-      for (int s = 1; s <= S - 2; ++s) x[s] = (de[s] - al[s] * first - ga[s] * U) / be[s];
+      for (int s = 1; s <= S - 2; ++s) x[s] = pdiv(de[s] - al[s] * first - ga[s] * U, be[s]);
     }
   }
 }
@@ -457,6 +472,26 @@ __global__ void __launch_bounds__(kHeatTPB) heat_sweep_x_wave(HeatArgs<double> a
     T[s] = k < nx ? T1[i0 + k] : 0.0;
     Mv[s] = k < nx ? (int)Mg[i0 + k] : 0;
   }
+  // neighbouring lines and the source, loaded up front (clamped addresses)
+  double Tym[S], Typ[S], Tzm[S], Tzp[S];
+  int Mym[S], Myp[S], Mzm[S], Mzp[S];
+  float Qv[S];
+#pragma unroll
+  for (int s = 0; s < S; ++s) {
+    const int k = lane * S + s;
+    const int i = i0 + (k < nx ? k : 0);
+    const int iym = y != 0 ? i - nx : i, iyp = y != ny - 1 ? i + nx : i;
+    const int izm = z != 0 ? i - nxy : i, izp = z != nz - 1 ? i + nxy : i;
+    Tym[s] = T1[iym];
+    Typ[s] = T1[iyp];
+    Tzm[s] = T1[izm];
+    Tzp[s] = T1[izp];
+    Mym[s] = Mg[iym];
+    Myp[s] = Mg[iyp];
+    Mzm[s] = Mg[izm];
+    Mzp[s] = Mg[izp];
+    Qv[s] = Q[i];
+  }
   const double Tl = lane_shfl(T[S - 1], lane - 1), Tr = lane_shfl(T[0], lane + 1);
   const int Ml = __shfl(Mv[S - 1], lane > 0 ? lane - 1 : 0, 64), Mr = __shfl(Mv[0], lane < 63 ? lane + 1 : 63, 64);
   double ca[S], cb[S], cc_[S], cd[S], xs[S];
@@ -468,7 +503,6 @@ __global__ void __launch_bounds__(kHeatTPB) heat_sweep_x_wave(HeatArgs<double> a
     cc_[s] = 0.0;
     cd[s] = 0.0;
     if (k >= nx) continue;
-    const int i = i0 + k;
     const int m = Mv[s];
     const double TL = s > 0 ? T[s - 1] : Tl, TR = s < S - 1 ? T[s + 1] : Tr;
     const int ML = s > 0 ? Mv[s - 1] : Ml, MR = s < S - 1 ? Mv[s + 1] : Mr;
@@ -477,13 +511,13 @@ __global__ void __launch_bounds__(kHeatTPB) heat_sweep_x_wave(HeatArgs<double> a
       cd[s] = T[s];
       continue;
     }
-    double dTdt = a.lightsOn ? (double)Q[i] : 0.0;
+    double dTdt = a.lightsOn ? (double)Qv[s] : 0.0;
     if (k != 0) dTdt += (TL - T[s]) * coef(cT, nM, m, ML, 0) / 2;
     if (k != nx - 1) dTdt += (TR - T[s]) * coef(cT, nM, m, MR, 0) / 2;
-    if (y != 0) dTdt += (T1[i - nx] - T[s]) * coef(cT, nM, m, (int)Mg[i - nx], 1);
-    if (y != ny - 1) dTdt += (T1[i + nx] - T[s]) * coef(cT, nM, m, (int)Mg[i + nx], 1);
-    if (z != 0) dTdt += (T1[i - nxy] - T[s]) * coef(cT, nM, m, (int)Mg[i - nxy], 2);
-    if (z != nz - 1) dTdt += (T1[i + nxy] - T[s]) * coef(cT, nM, m, (int)Mg[i + nxy], 2);
+    if (y != 0) dTdt += (Tym[s] - T[s]) * coef(cT, nM, m, Mym[s], 1);
+    if (y != ny - 1) dTdt += (Typ[s] - T[s]) * coef(cT, nM, m, Myp[s], 1);
+    if (z != 0) dTdt += (Tzm[s] - T[s]) * coef(cT, nM, m, Mzm[s], 2);
+    if (z != nz - 1) dTdt += (Tzp[s] - T[s]) * coef(cT, nM, m, Mzp[s], 2);
     cd[s] = T[s] + dt * dTdt;
     if (k < nx - 1) {
       const double q = dt / 2 * coef(cT, nM, m, MR, 0);
@@ -506,51 +540,86 @@ __global__ void __launch_bounds__(kHeatTPB) heat_sweep_x_wave(HeatArgs<double> a
 }
 
 // substeps 2a + 2b (KIND Y) and 3a + 3b with the Arrhenius update and the
-// per-medium maxima (KIND Z) in FP64 with one wavefront per line: lane l
-// holds voxels l*S .. l*S+S-1 of the line.  Consecutive wavefronts take
-// consecutive x, so the strided loads of a workgroup share cache lines.
-// The right-hand side is T1 of the previous substep minus dt/2 L_d T(step
-// start) along the line, in the expression order of the sequential kernel.
+// per-medium maxima (KIND Z) in FP64, lines staged through LDS: a workgroup
+// takes TX consecutive x-lines at one z (Y) or y (Z), loads them row by row
+// (TX consecutive x per row, coalesced; all of a thread's loads issued before
+// its first LDS store) into LDS with each line contiguous, each wavefront
+// solves TX/4 lines from LDS with the wavefront solver (lane l holds voxels
+// l*S .. l*S+S-1), and the workgroup writes the rows back.  The right-hand
+// side is T1 of the previous substep minus dt/2 L_d T(step start) along the
+// line, in the expression order of the sequential kernel.
+template <int S>
+struct YzTile {
+  static constexpr int TX = S <= 4 ? 16 : 8;  // lines per workgroup
+  static constexpr int NP = 64 * S;           // padded line length
+  static constexpr int LD = NP + 1;           // LDS pitch of a line (doubles)
+};
+
 template <int S, int KIND>
-__global__ void __launch_bounds__(kHeatTPB) heat_sweep_yz_wave(HeatArgs<double> a) {
-  __shared__ double sc[kLdsCoef];
-  __shared__ double sLogA[256], sE[256];
-  __shared__ unsigned long long sMax[256];
-  const int nc = a.nM * a.nM * 3;
-  if (nc <= kLdsCoef)
-    for (int k = threadIdx.x; k < nc; k += blockDim.x) sc[k] = a.c[k];
+__global__ void __launch_bounds__(kHeatTPB) heat_sweep_yz_tile(HeatArgs<double> a) {
+  constexpr int TX = YzTile<S>::TX, NP = YzTile<S>::NP, LD = YzTile<S>::LD;
+  __shared__ double sT[TX * LD], sR[TX * LD];
+  __shared__ uint8_t sM[TX * NP];
+  __shared__ double sLogA[KIND == SWEEP_Z ? 256 : 1], sE[KIND == SWEEP_Z ? 256 : 1];
+  __shared__ unsigned long long sMax[KIND == SWEEP_Z ? 256 : 1];
+  const int nx = a.nx, ny = a.ny, nz = a.nz, nxy = nx * ny, nM = a.nM;
+  const int n = KIND == SWEEP_Y ? ny : nz;
+  const int st = KIND == SWEEP_Y ? nx : nxy;
+  const int d = KIND == SWEEP_Y ? 1 : 2;
+  const int nOther = KIND == SWEEP_Y ? nz : ny;
+  const int ntx = (nx + TX - 1) / TX;
+  const int tx0 = (blockIdx.x % ntx) * TX, o = blockIdx.x / ntx;
+  const int base = KIND == SWEEP_Y ? tx0 + o * nxy : tx0 + o * nx;
+  const int cols = min(TX, nx - tx0);
+  const double* __restrict__ T1 = a.T1;
+  const uint8_t* __restrict__ Mg = a.M;
   if (KIND == SWEEP_Z)
-    for (int k = threadIdx.x; k < a.nM; k += blockDim.x) {
+    for (int k = threadIdx.x; k < nM; k += blockDim.x) {
       sLogA[k] = a.logA[k];
       sE[k] = a.E[k];
       sMax[k] = order_key(-INFINITY);
     }
+  // all loads of a thread are issued before the first LDS store
+  constexpr int kIt = (TX * NP + kHeatTPB - 1) / kHeatTPB;
+  {
+    double vT[kIt], vR[kIt];
+    int vM[kIt];
+#pragma unroll
+    for (int q = 0; q < kIt; ++q) {
+      const int e = threadIdx.x + q * kHeatTPB;
+      const int c = e % TX, k = e / TX;
+      const bool ok = c < cols && k < n;
+      const int i = ok ? base + c + k * st : base;
+      vT[q] = T1[i];
+      vR[q] = a.T2[i];
+      vM[q] = Mg[i];
+    }
+#pragma unroll
+    for (int q = 0; q < kIt; ++q) {
+      const int e = threadIdx.x + q * kHeatTPB;
+      const int c = e % TX, k = e / TX;
+      if (c < cols && k < n) {
+        sT[c * LD + k] = vT[q];
+        sR[c * LD + k] = vR[q];
+        sM[c * NP + k] = (uint8_t)vM[q];
+      }
+    }
+  }
   __syncthreads();
-  const double* __restrict__ cT = nc <= kLdsCoef ? sc : a.c;
-  const int lane = __lane_id();
-  const int line = blockIdx.x * (kHeatTPB / 64) + (threadIdx.x >> 6);
-  const int nx = a.nx, ny = a.ny, nz = a.nz, nxy = nx * ny, nM = a.nM;
-  const int nLines = KIND == SWEEP_Y ? nx * nz : nxy;
+  const int lane = __lane_id(), wave = threadIdx.x >> 6;
   const bool sk = a.sinked;
   const double dt = a.dt;
-  if (line < nLines) {
-    const int x = line % nx, o = line / nx;  // o = z (Y) or y (Z)
-    const int i0 = KIND == SWEEP_Y ? x + o * nxy : x + o * nx;
-    const int st = KIND == SWEEP_Y ? nx : nxy;
-    const int n = KIND == SWEEP_Y ? ny : nz;
-    const int d = KIND == SWEEP_Y ? 1 : 2;
-    const bool lf = sk && (x == 0 || x == nx - 1 || o == 0 || o == (KIND == SWEEP_Y ? nz : ny) - 1);
-    const double* __restrict__ T1 = a.T1;
-    const uint8_t* __restrict__ Mg = a.M;
+  for (int c = wave; c < cols; c += kHeatTPB / 64) {
+    const int x = tx0 + c;
+    const bool lf = sk && (x == 0 || x == nx - 1 || o == 0 || o == nOther - 1);
     double Tn[S], R0[S];
     int Mv[S];
 #pragma unroll
     for (int s = 0; s < S; ++s) {
       const int k = lane * S + s;
-      const int i = i0 + k * st;
-      Tn[s] = k < n ? T1[i] : 0.0;
-      R0[s] = k < n ? a.T2[i] : 0.0;
-      Mv[s] = k < n ? (int)Mg[i] : 0;
+      Tn[s] = k < n ? sT[c * LD + k] : 0.0;
+      R0[s] = k < n ? sR[c * LD + k] : 0.0;
+      Mv[s] = k < n ? (int)sM[c * NP + k] : 0;
     }
     const double Tl = lane_shfl(Tn[S - 1], lane - 1), Tr = lane_shfl(Tn[0], lane + 1);
     const int Ml = __shfl(Mv[S - 1], lane > 0 ? lane - 1 : 0, 64), Mr = __shfl(Mv[0], lane < 63 ? lane + 1 : 63, 64);
@@ -570,16 +639,16 @@ __global__ void __launch_bounds__(kHeatTPB) heat_sweep_yz_wave(HeatArgs<double> 
       double t = R0[s];
       if (!fz) {
         double dTdt = 0;
-        if (k != 0) dTdt -= (TL - Tn[s]) * coef(cT, nM, m, ML, d) / 2;
-        if (k != n - 1) dTdt -= (TR - Tn[s]) * coef(cT, nM, m, MR, d) / 2;
+        if (k != 0) dTdt -= (TL - Tn[s]) * cc(a, m, ML, d) / 2;
+        if (k != n - 1) dTdt -= (TR - Tn[s]) * cc(a, m, MR, d) / 2;
         t = t + dt * dTdt;
         if (k < n - 1) {
-          const double q = dt / 2 * coef(cT, nM, m, MR, d);
+          const double q = dt / 2 * cc(a, m, MR, d);
           cb[s] += q;
           cc_[s] = -q;
         }
         if (k > 0) {
-          const double q = dt / 2 * coef(cT, nM, m, ML, d);
+          const double q = dt / 2 * cc(a, m, ML, d);
           cb[s] += q;
           ca[s] = -q;
         }
@@ -587,23 +656,44 @@ __global__ void __launch_bounds__(kHeatTPB) heat_sweep_yz_wave(HeatArgs<double> 
       cd[s] = t;
     }
     wave_tridiag<S>(ca, cb, cc_, cd, xs);
-    double* __restrict__ T2 = a.T2;
-    MaxRun run;
 #pragma unroll
     for (int s = 0; s < S; ++s) {
       const int k = lane * S + s;
-      if (k >= n) continue;
-      const int i = i0 + k * st;
-      T2[i] = xs[s];
-      if (KIND == SWEEP_Z) {
-        const int m = Mv[s];
-        if (a.calcDamage && sE[m] != 0) a.Om[i] += dt * exp(sLogA[m] - sE[m] / (a.Rgas * ((xs[s] + Tn[s]) / 2 + a.czero)));
-        run.add(m, xs[s], sMax);
-      }
+      if (k < n) sR[c * LD + k] = xs[s];
     }
-    if (KIND == SWEEP_Z) run.flush(sMax);
   }
-  if (KIND == SWEEP_Z) fold_lds_max(sMax, a.maxKey, a.nM);
+  __syncthreads();
+  MaxRun run;
+  double* __restrict__ T2 = a.T2;
+  double* __restrict__ Om = a.Om;
+  double vO[KIND == SWEEP_Z ? kIt : 1];
+  if (KIND == SWEEP_Z && a.calcDamage) {
+#pragma unroll
+    for (int q = 0; q < kIt; ++q) {
+      const int e = threadIdx.x + q * kHeatTPB;
+      const int c = e % TX, k = e / TX;
+      const bool ok = c < cols && k < n;
+      vO[q] = Om[ok ? base + c + k * st : base];
+    }
+  }
+#pragma unroll
+  for (int q = 0; q < kIt; ++q) {
+    const int e = threadIdx.x + q * kHeatTPB;
+    const int c = e % TX, k = e / TX;
+    if (c >= cols || k >= n) continue;
+    const int i = base + c + k * st;
+    const double t = sR[c * LD + k];
+    T2[i] = t;
+    if (KIND == SWEEP_Z) {
+      const int m = sM[c * NP + k];
+      if (a.calcDamage && sE[m] != 0) Om[i] = vO[q] + dt * exp(sLogA[m] - sE[m] / (a.Rgas * ((t + sT[c * LD + k]) / 2 + a.czero)));
+      run.add(m, t, sMax);
+    }
+  }
+  if (KIND == SWEEP_Z) {
+    run.flush(sMax);
+    fold_lds_max(sMax, a.maxKey, nM);
+  }
 }
 
 // rows per lane of the wavefront x-solve for a line of n voxels (0: use the
@@ -951,23 +1041,29 @@ void launch_sweep_x(mcm_heat_plan* P, const HeatArgs<RT>& a, unsigned gx) {
 
 // y and z sweeps: the wavefront solver for FP64 lines of up to 512 voxels,
 // else the sequential lane-per-line kernels.
+template <int S, int KIND>
+void launch_yz_tile(mcm_heat_plan* P, const HeatArgs<double>& a) {
+  constexpr int TX = YzTile<S>::TX;
+  const unsigned g = (unsigned)(((P->nx + TX - 1) / TX) * (KIND == SWEEP_Y ? P->nz : P->ny));
+  hipLaunchKernelGGL((heat_sweep_yz_tile<S, KIND>), dim3(g), dim3(kHeatTPB), 0, P->stream, a);
+}
+
 template <int KIND>
-void launch_yz_wave(mcm_heat_plan* P, const HeatArgs<double>& a, int n, int64_t lines) {
-  const unsigned gw = (unsigned)((lines + kHeatTPB / 64 - 1) / (kHeatTPB / 64));
+void launch_yz_wave(mcm_heat_plan* P, const HeatArgs<double>& a, int n) {
   switch (wave_rows(n)) {
-    case 1: hipLaunchKernelGGL((heat_sweep_yz_wave<1, KIND>), dim3(gw), dim3(kHeatTPB), 0, P->stream, a); break;
-    case 2: hipLaunchKernelGGL((heat_sweep_yz_wave<2, KIND>), dim3(gw), dim3(kHeatTPB), 0, P->stream, a); break;
-    case 4: hipLaunchKernelGGL((heat_sweep_yz_wave<4, KIND>), dim3(gw), dim3(kHeatTPB), 0, P->stream, a); break;
-    default: hipLaunchKernelGGL((heat_sweep_yz_wave<8, KIND>), dim3(gw), dim3(kHeatTPB), 0, P->stream, a); break;
+    case 1: launch_yz_tile<1, KIND>(P, a); return;
+    case 2: launch_yz_tile<2, KIND>(P, a); return;
+    case 4: launch_yz_tile<4, KIND>(P, a); return;
+    default: launch_yz_tile<8, KIND>(P, a); return;
   }
 }
 
 template <typename RT>
 void launch_sweep_yz(mcm_heat_plan* P, const HeatArgs<RT>& a, unsigned gy, unsigned gz) {
   if constexpr (std::is_same<RT, double>::value) {
-    if (wave_rows(P->ny)) launch_yz_wave<SWEEP_Y>(P, a, P->ny, (int64_t)P->nx * P->nz);
+    if (wave_rows(P->ny)) launch_yz_wave<SWEEP_Y>(P, a, P->ny);
     else hipLaunchKernelGGL(heat_sweep_y<RT>, dim3(gy), dim3(kHeatTPB), 0, P->stream, a);
-    if (wave_rows(P->nz)) launch_yz_wave<SWEEP_Z>(P, a, P->nz, (int64_t)P->nx * P->ny);
+    if (wave_rows(P->nz)) launch_yz_wave<SWEEP_Z>(P, a, P->nz);
     else hipLaunchKernelGGL(heat_sweep_z<RT>, dim3(gz), dim3(kHeatTPB), 0, P->stream, a);
     return;
   }
