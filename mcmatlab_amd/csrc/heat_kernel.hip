@@ -892,6 +892,83 @@ __global__ void __launch_bounds__(kHeatTPB) heat_adj_solve(HeatArgs<double> a, c
   }
 }
 
+// A_d^T y = r with one wavefront per line (lines of up to 512 voxels): the
+// rows of A^T are (sub, diag, super) = (c_(k-1), b_k, a_(k+1)) of A's rows.
+template <int S, int AXIS>
+__global__ void __launch_bounds__(kHeatTPB) heat_adj_solve_wave(HeatArgs<double> a, const double* __restrict__ r, double* y) {
+  const int nx = a.nx, ny = a.ny, nz = a.nz, nxy = nx * ny, nM = a.nM;
+  const int lane = __lane_id();
+  const int line = blockIdx.x * (kHeatTPB / 64) + (threadIdx.x >> 6);
+  int i0, st, n, nLines;
+  bool lf;
+  if (AXIS == 0) {
+    nLines = ny * nz;
+    const int yy = line % ny, zz = line / ny;
+    i0 = yy * nx + zz * nxy, st = 1, n = nx;
+    lf = a.sinked && (yy == 0 || yy == ny - 1 || zz == 0 || zz == nz - 1);
+  } else if (AXIS == 1) {
+    nLines = nx * nz;
+    const int xx = line % nx, zz = line / nx;
+    i0 = xx + zz * nxy, st = nx, n = ny;
+    lf = a.sinked && (xx == 0 || xx == nx - 1 || zz == 0 || zz == nz - 1);
+  } else {
+    nLines = nxy;
+    i0 = line, st = nxy, n = nz;
+    const int xx = line % nx, yy = line / nx;
+    lf = a.sinked && (xx == 0 || xx == nx - 1 || yy == 0 || yy == ny - 1);
+  }
+  if (line >= nLines) return;
+  const double h = a.dt / 2;
+  double rv[S], A[S], B[S], Cc[S];
+  int Mv[S];
+#pragma unroll
+  for (int s = 0; s < S; ++s) {
+    const int k = lane * S + s;
+    const int i = i0 + (k < n ? k : 0) * st;
+    rv[s] = k < n ? r[i] : 0.0;
+    Mv[s] = a.M[i];
+  }
+  const int Ml = __shfl(Mv[S - 1], lane > 0 ? lane - 1 : 0, 64), Mr = __shfl(Mv[0], lane < 63 ? lane + 1 : 63, 64);
+  // rows of A
+#pragma unroll
+  for (int s = 0; s < S; ++s) {
+    const int k = lane * S + s;
+    A[s] = 0.0;
+    B[s] = 1.0;
+    Cc[s] = 0.0;
+    if (k >= n || lf || (a.sinked && (k == 0 || k == n - 1))) continue;
+    const int m = Mv[s];
+    const int ML = s > 0 ? Mv[s - 1] : Ml, MR = s < S - 1 ? Mv[s + 1] : Mr;
+    if (k > 0) {
+      const double q = h * cc(a, m, ML, AXIS);
+      B[s] += q;
+      A[s] = -q;
+    }
+    if (k < n - 1) {
+      const double q = h * cc(a, m, MR, AXIS);
+      B[s] += q;
+      Cc[s] = -q;
+    }
+  }
+  // rows of A^T
+  const double cPrevLane = lane_shfl(Cc[S - 1], lane - 1), aNextLane = lane_shfl(A[0], lane + 1);
+  double ta[S], tb[S], tc[S], xs[S];
+#pragma unroll
+  for (int s = 0; s < S; ++s) {
+    const int k = lane * S + s;
+    const bool ok = k < n;
+    ta[s] = ok && k > 0 ? (s > 0 ? Cc[s - 1] : cPrevLane) : 0.0;
+    tb[s] = ok ? B[s] : 1.0;
+    tc[s] = ok && k < n - 1 ? (s < S - 1 ? A[s + 1] : aNextLane) : 0.0;
+  }
+  wave_tridiag<S>(ta, tb, tc, rv, xs);
+#pragma unroll
+  for (int s = 0; s < S; ++s) {
+    const int k = lane * S + s;
+    if (k < n) y[i0 + k * st] = xs[s];
+  }
+}
+
 __device__ inline bool heat_frozen(const HeatArgs<double>& a, int x, int y, int z) { return frozen(a, x, y, z); }
 
 // lam_u of the step (plus the damage adjoint of u) and the source adjoint.
@@ -1165,6 +1242,22 @@ int download(mcm_heat_plan* P, mcm_heat_output* out) {
   return MCM_OK;
 }
 
+// A_d^T solve along axis AXIS: the wavefront solver for lines of up to 512
+// voxels, else the lane-per-line recurrence.
+template <int AXIS>
+void adj_solve(mcm_heat_plan* P, const HeatArgs<double>& a, const double* r, double* y, double* beta, unsigned gLane) {
+  const int n = AXIS == 0 ? P->nx : AXIS == 1 ? P->ny : P->nz;
+  const int64_t lines = AXIS == 0 ? (int64_t)P->ny * P->nz : AXIS == 1 ? (int64_t)P->nx * P->nz : (int64_t)P->nx * P->ny;
+  const unsigned gw = (unsigned)((lines + kHeatTPB / 64 - 1) / (kHeatTPB / 64));
+  switch (wave_rows(n)) {
+    case 1: hipLaunchKernelGGL((heat_adj_solve_wave<1, AXIS>), dim3(gw), dim3(kHeatTPB), 0, P->stream, a, r, y); return;
+    case 2: hipLaunchKernelGGL((heat_adj_solve_wave<2, AXIS>), dim3(gw), dim3(kHeatTPB), 0, P->stream, a, r, y); return;
+    case 4: hipLaunchKernelGGL((heat_adj_solve_wave<4, AXIS>), dim3(gw), dim3(kHeatTPB), 0, P->stream, a, r, y); return;
+    case 8: hipLaunchKernelGGL((heat_adj_solve_wave<8, AXIS>), dim3(gw), dim3(kHeatTPB), 0, P->stream, a, r, y); return;
+    default: hipLaunchKernelGGL(heat_adj_solve<AXIS>, dim3(gLane), dim3(kHeatTPB), 0, P->stream, a, r, y, beta); return;
+  }
+}
+
 }  // namespace
 
 extern "C" {
@@ -1289,6 +1382,7 @@ int mcm_heat_plan_destroy(mcm_heat_plan* P) {
   return MCM_OK;
 }
 
+
 int mcm_heat_adjoint_run(const mcm_heat_input* in, const mcm_heat_segment* segments, int64_t nSegments, const uint8_t* targetMask,
                          int64_t checkpointEvery, mcm_heat_adjoint_output* out) {
   if (!in || !out) return fail(MCM_ERR_INVALID_ARGUMENT, "null argument");
@@ -1367,9 +1461,9 @@ int mcm_heat_adjoint_run(const mcm_heat_input* in, const mcm_heat_segment* segme
       const double* x1 = segp(n - s0 + 1);
       double* B = (double*)P->B.p;
       hipLaunchKernelGGL(heat_adj_damage, dim3(gv), dim3(tpb), 0, s, a, x0, x1, (const uint8_t*)dMask.p, (double*)lam.p, L);
-      hipLaunchKernelGGL(heat_adj_solve<2>, dim3(gz), dim3(tpb), 0, s, a, (const double*)lam.p, (double*)mu3.p, B);
-      hipLaunchKernelGGL(heat_adj_solve<1>, dim3(gy), dim3(tpb), 0, s, a, (const double*)mu3.p, (double*)mu2.p, B);
-      hipLaunchKernelGGL(heat_adj_solve<0>, dim3(gx), dim3(tpb), 0, s, a, (const double*)mu2.p, (double*)mu1.p, B);
+      adj_solve<2>(P, a, (const double*)lam.p, (double*)mu3.p, B, gz);
+      adj_solve<1>(P, a, (const double*)mu3.p, (double*)mu2.p, B, gy);
+      adj_solve<0>(P, a, (const double*)mu2.p, (double*)mu1.p, B, gx);
       hipLaunchKernelGGL(heat_adj_combine, dim3(gv), dim3(tpb), 0, s, a, (const double*)mu1.p, (const double*)mu2.p, (const double*)mu3.p, x0,
                          x1, (const uint8_t*)dMask.p, (double*)lamN.p, (double*)dQ.p, lights[(size_t)n]);
       MCM_HIP(hipGetLastError());

@@ -38,7 +38,9 @@ def case(bt=1, n=16, shape=(14, 12, 10), seed=0):
     P["dTdtperdeltaT"] = heat_coefficients([1.2e-3, 3.7, 3.9, 3.8], [2.6e-4, 0.0037, 0.0052, 0.0045], d, d, d)
     nx, ny, nz = shape
     rng = np.random.default_rng(seed)
-    M = np.asarray(P["M"])[:nx, :ny, :nz].copy(order="F")
+    M0 = np.asarray(P["M"])
+    reps = [-(-e // m) for e, m in zip(shape, M0.shape)]
+    M = np.tile(M0, reps)[:nx, :ny, :nz].copy(order="F")
     M[rng.random(M.shape) < 0.25] = 3  # more blood (E != 0)
     P["M"] = M
     P["dTdt_abs"] = (3e4 * rng.random((nx, ny, nz))).astype(np.float32)
@@ -90,6 +92,24 @@ def test_adjoint_dQ_matches_central_differences(api, bt):
     # dJ / ad = 1 + c h^2 + O(h^4) for central differences: Richardson
     rich = (4 * (dJ2 / ad2) - dJ1 / ad1) / 3
     assert abs(rich - 1) <= 1e-4, (dJ1 / ad1, dJ2 / ad2, rich)
+
+
+@pytest.mark.parametrize("shape", [(520, 4, 3), (3, 600, 4), (4, 3, 513), (5, 512, 3)])
+def test_adjoint_long_lines(api, shape):
+    """Lines beyond one wavefront's reach (more than 512 voxels) take the
+    lane-per-line recurrence; 512 is the longest wavefront solve."""
+    T0, Om0, P, segs = case(1, shape=shape, seed=5)
+    segs = [(6, 2e-5, True), (3, 3e-5, False)]
+    r = api.heat_adjoint(T0, Om0, P, segments=segs, checkpointEvery=4)
+    rng = np.random.default_rng(2)
+    d = rng.standard_normal(T0.shape)
+    h = 1e-3
+    base = T0.astype(np.float64)
+    Jp = api.heat_adjoint(T0, Om0, P, segments=segs, grads=False, T_f64=base + h * d)["J"]
+    Jm = api.heat_adjoint(T0, Om0, P, segments=segs, grads=False, T_f64=base - h * d)["J"]
+    fd = (Jp - Jm) / (2 * h)
+    ad = float(np.sum(r["dJ_dT0"] * d))
+    assert abs(fd - ad) <= 1e-6 * abs(ad) + 1e-12 * r["J"], (fd, ad)
 
 
 def test_adjoint_lights_off_has_no_source_gradient(api):
