@@ -260,7 +260,25 @@ int mcm_heat_plan_create(const mcm_heat_input *in, mcm_heat_plan **plan);
 /* Advance `steps` steps of size dt with the given illumination; async on the
  * plan's stream.  Sensors are recorded into the plan's device buffer. */
 int mcm_heat_plan_advance(mcm_heat_plan *plan, int64_t steps, double dt, int32_t lightsOn);
+/* Runs of equal (dt, lightsOn) steps are replayed as captured HIP graphs of
+ * `steps` (even) time steps each, one launch per graph instead of 2-4 per
+ * step; 0 launches every step.  Default 32.  No reference counterpart (the
+ * reference launches its kernels step by step,
+ * finiteElementHeatPropagator.c:764-798). */
+int mcm_heat_plan_set_graph_steps(mcm_heat_plan *plan, int64_t steps);
 int mcm_heat_plan_download(mcm_heat_plan *plan, mcm_heat_output *out);
+
+/* Several plans (different grids and models, one device) advanced together:
+ * runs of stepsPerGraph (even) steps of every plan are captured as one graph
+ * whose branches (one per plan, on the plan's stream) run concurrently, and
+ * replayed; remaining steps are launched per plan.  dt[i] and lightsOn[i]
+ * are plan i's.  Results equal advancing each plan alone.  A plan stays
+ * usable on its own between group calls; destroy the group before its plans.
+ * No reference counterpart (batched sweeps of independent models). */
+typedef struct mcm_heat_group mcm_heat_group;
+int mcm_heat_group_create(mcm_heat_plan *const *plans, int64_t nPlans, mcm_heat_group **group);
+int mcm_heat_group_advance(mcm_heat_group *group, int64_t steps, const double *dt, const int32_t *lightsOn, int64_t stepsPerGraph);
+int mcm_heat_group_destroy(mcm_heat_group *group);
 int mcm_heat_plan_destroy(mcm_heat_plan *plan);
 
 /* Discrete adjoint of the FP64 DG-ADI propagation (no reference

@@ -366,14 +366,14 @@ class HeatInputs:
 
 
 class HeatOutputs:
-    def __init__(self, inp: HeatInputs, want_f64=False):
+    def __init__(self, inp: HeatInputs, want_f64=False, steps=None):
         s = inp.struct
         L = int(np.prod(inp.shape))
         self.shape = inp.shape
         self.T = np.zeros(L, np.float32)
         self.Omega = np.zeros(L if inp.calcDamage else 1, np.float32)
         self.nS = int(s.nSensors)
-        self.steps = int(s.steps)
+        self.steps = int(s.steps) if steps is None else int(steps)
         self.sensor = np.zeros(max(self.nS * (self.steps + 1), 1), np.float64)
         self.maxT = np.zeros(int(s.nM), np.float32)
         o = mcm_heat_output()

@@ -88,6 +88,92 @@ def finiteElementHeatPropagator(T, Omega, P, precision="fp64", scheme="dgadi", s
 finiteElementHeatPropagator.last_state = (None, None)
 
 
+class HeatPlan:
+    """Device-resident heat propagation (mcm_heat_plan_*): upload T, Omega
+    and the model once, advance any schedule of (steps, dt, lightsOn) runs,
+    download when needed.  Runs of 2 * graph_steps steps or more are replayed
+    as captured HIP graphs."""
+
+    def __init__(self, T, Omega, P, precision="fp64", scheme="dgadi", graph_steps=None):
+        prec = {"fp64": abi.MCM_HEAT_FP64, "fp32": abi.MCM_HEAT_FP32}[precision]
+        sch = {"dgadi": abi.MCM_HEAT_DGADI, "explicit": abi.MCM_HEAT_EXPLICIT}[scheme]
+        self._inp = abi.HeatInputs(T, Omega, P, precision=prec, scheme=sch)
+        h = C.c_void_p()
+        check(lib().mcm_heat_plan_create(C.byref(self._inp.struct), C.byref(h)))
+        self.h = h
+        self.steps = 0
+        if graph_steps is not None:
+            self.set_graph_steps(graph_steps)
+
+    def set_graph_steps(self, k):
+        check(lib().mcm_heat_plan_set_graph_steps(self.h, int(k)))
+
+    def advance(self, steps, dt, lightsOn):
+        check(lib().mcm_heat_plan_advance(self.h, int(steps), float(dt), int(bool(lightsOn))))
+        self.steps += int(steps)
+
+    def download(self, f64=True):
+        out = abi.HeatOutputs(self._inp, want_f64=f64, steps=self.steps)
+        check(lib().mcm_heat_plan_download(self.h, C.byref(out.struct)))
+        return out.result()
+
+    def close(self):
+        if self.h:
+            lib().mcm_heat_plan_destroy(self.h)
+            self.h = None
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+class HeatGroup:
+    """HeatPlans advanced together (mcm_heat_group_*): every run of
+    steps_per_graph steps of all plans is one graph launch whose per-plan
+    branches run concurrently.  The plans stay owned by the caller."""
+
+    def __init__(self, plans, steps_per_graph=32):
+        self.plans = list(plans)
+        arr = (C.c_void_p * len(self.plans))(*[p.h.value for p in self.plans])
+        h = C.c_void_p()
+        check(lib().mcm_heat_group_create(arr, len(self.plans), C.byref(h)))
+        self.h = h
+        self.k = int(steps_per_graph)
+
+    def advance(self, steps, dts, lights):
+        n = len(self.plans)
+        d = (C.c_double * n)(*[float(x) for x in dts])
+        l = (C.c_int32 * n)(*[int(bool(x)) for x in lights])
+        check(lib().mcm_heat_group_advance(self.h, int(steps), d, l, self.k))
+        for p in self.plans:
+            p.steps += int(steps)
+
+    def close(self):
+        if self.h:
+            lib().mcm_heat_group_destroy(self.h)
+            self.h = None
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
 def heat_adjoint(T, Omega, P, segments=None, targetMask=None, checkpointEvery=0, grads=True, T_f64=None, Omega_f64=None):
     """mcm_heat_adjoint_run: final damage J summed over the target voxels
     (targetMask, default: media with E != 0) after the schedule `segments`

@@ -752,9 +752,11 @@ __global__ void __launch_bounds__(kHeatTPB) heat_explicit(HeatArgs<RT> a) {
 }
 
 // recordTempSensors (:652-663): trilinear interpolation of the new T into
-// column `col` of the sensor record
+// column `col` (plus *colBase when given) of the sensor record
 template <typename RT>
-__global__ void heat_sensors(const RT* T, int nx, int nxy, const long long* idx, const RT* w, int nS, double* out, long long col) {
+__global__ void heat_sensors(const RT* T, int nx, int nxy, const long long* idx, const RT* w, int nS, double* out, const long long* colBase,
+                             long long col) {
+  if (colBase) col += *colBase;
   for (int s = threadIdx.x; s < nS; s += blockDim.x) {
     const long long k = idx[s];
     const RT wx = w[s], wy = w[s + nS], wz = w[s + 2 * nS];
@@ -763,6 +765,12 @@ __global__ void heat_sensors(const RT* T, int nx, int nxy, const long long* idx,
                                  wx * (1 - wy) * (1 - wz) * T[k + 1] + wx * (1 - wy) * wz * T[k + 1 + nxy] +
                                  wx * wy * (1 - wz) * T[k + 1 + nx] + wx * wy * wz * T[k + 1 + nx + nxy]);
   }
+}
+
+// step counter of graph-replayed steps: set before a series of replays,
+// advanced by the last node of each replay
+__global__ void heat_step_counter(long long* c, long long value, int add) {
+  if (threadIdx.x == 0) *c = add ? *c + value : value;
 }
 
 template <typename RT>
@@ -1019,6 +1027,16 @@ struct HeatBuf {
   }
 };
 
+// An instantiated graph of `steps` time steps at one (dt, lights) setting,
+// valid while the state and sensor buffers it captured stay in place.
+struct HeatGraph {
+  hipGraphExec_t exec = nullptr;
+  int64_t steps = 0;
+  double dt = 0;
+  int lightsOn = 0;
+  const void *T1 = nullptr, *sensors = nullptr;
+};
+
 struct mcm_heat_plan {
   int device = 0;
   hipStream_t stream = nullptr;
@@ -1031,10 +1049,44 @@ struct mcm_heat_plan {
   int64_t sensorCap = 0;      // columns allocated in `sensors`
   bool interrupted = false;
   int (*interrupt)(void) = nullptr;
-  HeatBuf M, Q, c, logA, E, T1, T2, B, Om, maxKey, sIdx, sW, sensors, stage32, stage64;
+  HeatBuf M, Q, c, logA, E, T1, T2, B, Om, maxKey, sIdx, sW, sensors, stage32, stage64, stepDev;
   double constR = 0, constC = 0;
+  int64_t graphSteps = 32;    // steps per captured graph (0: launch every step)
+  std::vector<HeatGraph> graphs;
+  void drop_graphs() {
+    for (HeatGraph& g : graphs)
+      if (g.exec) (void)hipGraphExecDestroy(g.exec);
+    graphs.clear();
+  }
   ~mcm_heat_plan() {
+    drop_graphs();
     if (stream) (void)hipStreamDestroy(stream);
+  }
+};
+
+// Plans advanced together: one graph holds K steps of every plan, each plan
+// a branch on its own stream, so the small kernels of different grids run
+// concurrently.
+struct mcm_heat_group {
+  int device = 0;
+  std::vector<mcm_heat_plan*> plans;
+  std::vector<hipEvent_t> events;  // [0] fork, [1 + i] join of plan i
+  struct Graph {
+    hipGraphExec_t exec = nullptr;
+    int64_t steps = 0;
+    std::vector<double> dt;
+    std::vector<int> lightsOn;
+    std::vector<const void*> T1, sensors;
+  };
+  std::vector<Graph> graphs;
+  void drop_graphs() {
+    for (Graph& g : graphs)
+      if (g.exec) (void)hipGraphExecDestroy(g.exec);
+    graphs.clear();
+  }
+  ~mcm_heat_group() {
+    drop_graphs();
+    for (hipEvent_t e : events) (void)hipEventDestroy(e);
   }
 };
 
@@ -1092,7 +1144,8 @@ template <typename RT>
 int record_sensors(mcm_heat_plan* P, int64_t col) {
   if (P->nS == 0) return MCM_OK;
   hipLaunchKernelGGL(heat_sensors<RT>, dim3(1), dim3(256), 0, P->stream, (const RT*)P->T1.p, P->nx, P->nx * P->ny,
-                     (const long long*)P->sIdx.p, (const RT*)P->sW.p, (int)P->nS, (double*)P->sensors.p, (long long)col);
+                     (const long long*)P->sIdx.p, (const RT*)P->sW.p, (int)P->nS, (double*)P->sensors.p, (const long long*)nullptr,
+                     (long long)col);
   MCM_HIP(hipGetLastError());
   return MCM_OK;
 }
@@ -1148,29 +1201,103 @@ void launch_sweep_yz(mcm_heat_plan* P, const HeatArgs<RT>& a, unsigned gy, unsig
   hipLaunchKernelGGL(heat_sweep_z<RT>, dim3(gz), dim3(kHeatTPB), 0, P->stream, a);
 }
 
+// The kernels of one time step on the plan's stream; swaps the state buffers.
+template <typename RT>
+void enqueue_step(mcm_heat_plan* P, double dt, int lightsOn) {
+  const int tpb = kHeatTPB;
+  HeatArgs<RT> a = make_args<RT>(P, dt, lightsOn);
+  if (P->explicitScheme) {
+    const unsigned ge = (unsigned)(((int64_t)P->nx * P->ny * ((P->nz + kExplicitZ - 1) / kExplicitZ) + tpb - 1) / tpb);
+    hipLaunchKernelGGL(heat_explicit<RT>, dim3(ge), dim3(tpb), 0, P->stream, a);
+  } else {
+    launch_sweep_x<RT>(P, a, (unsigned)((P->ny * P->nz + tpb - 1) / tpb));
+    launch_sweep_yz<RT>(P, a, (unsigned)((P->nx * P->nz + tpb - 1) / tpb), (unsigned)((P->nx * P->ny + tpb - 1) / tpb));
+  }
+  std::swap(P->T1.p, P->T2.p);
+}
+
+// K (even) steps as graph nodes on the plan's stream, which is capturing:
+// sensor columns are taken from the device step counter, which the last
+// node advances by K.  An even K leaves T1/T2 where they were.
+template <typename RT>
+void enqueue_graph_steps(mcm_heat_plan* P, int64_t K, double dt, int lightsOn) {
+  for (int64_t k = 0; k < K; ++k) {
+    enqueue_step<RT>(P, dt, lightsOn);
+    if (P->nS)
+      hipLaunchKernelGGL(heat_sensors<RT>, dim3(1), dim3(256), 0, P->stream, (const RT*)P->T1.p, P->nx, P->nx * P->ny,
+                         (const long long*)P->sIdx.p, (const RT*)P->sW.p, (int)P->nS, (double*)P->sensors.p,
+                         (const long long*)P->stepDev.p, (long long)(k + 1));
+  }
+  if (P->nS) hipLaunchKernelGGL(heat_step_counter, dim3(1), dim3(64), 0, P->stream, (long long*)P->stepDev.p, (long long)K, 1);
+}
+
+// The cached graph of K steps at (dt, lightsOn) for the current buffers, or
+// a new capture (at most four are kept).
+template <typename RT>
+int step_graph(mcm_heat_plan* P, int64_t K, double dt, int lightsOn, hipGraphExec_t* out) {
+  for (HeatGraph& g : P->graphs)
+    if (g.steps == K && g.dt == dt && g.lightsOn == lightsOn && g.T1 == P->T1.p && g.sensors == P->sensors.p) {
+      *out = g.exec;
+      return MCM_OK;
+    }
+  if (P->nS && !P->stepDev.p) MCM_HIP(hipMalloc(&P->stepDev.p, sizeof(long long)));
+  MCM_HIP(hipStreamBeginCapture(P->stream, hipStreamCaptureModeThreadLocal));
+  enqueue_graph_steps<RT>(P, K, dt, lightsOn);
+  hipGraph_t graph = nullptr;
+  const hipError_t launchErr = hipGetLastError();
+  const hipError_t endErr = hipStreamEndCapture(P->stream, &graph);
+  if (launchErr != hipSuccess || endErr != hipSuccess) {
+    if (graph) (void)hipGraphDestroy(graph);
+    return fail(MCM_ERR_DEVICE, "capturing %lld heat steps: %s", (long long)K, hipGetErrorString(launchErr != hipSuccess ? launchErr : endErr));
+  }
+  hipGraphExec_t exec = nullptr;
+  const hipError_t instErr = hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0);
+  (void)hipGraphDestroy(graph);
+  if (instErr != hipSuccess) return fail(MCM_ERR_DEVICE, "instantiating the heat step graph: %s", hipGetErrorString(instErr));
+  if (P->graphs.size() >= 4) {
+    (void)hipGraphExecDestroy(P->graphs.front().exec);
+    P->graphs.erase(P->graphs.begin());
+  }
+  HeatGraph g;
+  g.exec = exec;
+  g.steps = K;
+  g.dt = dt;
+  g.lightsOn = lightsOn;
+  g.T1 = P->T1.p;
+  g.sensors = P->sensors.p;
+  P->graphs.push_back(g);
+  *out = exec;
+  return MCM_OK;
+}
+
 template <typename RT>
 int advance(mcm_heat_plan* P, int64_t steps, double dt, int lightsOn) {
   int rc = ensure_sensor_capacity(P, P->stepsDone + steps + 1);
   if (rc) return rc;
-  const int tpb = kHeatTPB;
-  const unsigned gx = (unsigned)((P->ny * P->nz + tpb - 1) / tpb);
-  const unsigned gy = (unsigned)((P->nx * P->nz + tpb - 1) / tpb);
-  const unsigned gz = (unsigned)((P->nx * P->ny + tpb - 1) / tpb);
-  for (int64_t n = 0; n < steps; ++n) {
+  int64_t n = 0;
+  // runs of K steps replay one captured graph (one launch instead of 2-4
+  // per step); the interrupt is polled between replays
+  const int64_t K = P->graphSteps & ~(int64_t)1;
+  if (K >= 2 && steps >= 2 * K) {
+    hipGraphExec_t g = nullptr;
+    if ((rc = step_graph<RT>(P, K, dt, lightsOn, &g))) return rc;
+    if (P->nS) hipLaunchKernelGGL(heat_step_counter, dim3(1), dim3(64), 0, P->stream, (long long*)P->stepDev.p, (long long)P->stepsDone, 0);
+    for (; steps - n >= K; n += K) {
+      if (P->interrupt && P->interrupt()) {
+        P->interrupted = true;
+        return MCM_OK;
+      }
+      MCM_HIP(hipGraphLaunch(g, P->stream));
+      P->stepsDone += K;
+    }
+  }
+  for (; n < steps; ++n) {
     if (P->interrupt && P->interrupt()) {
       P->interrupted = true;
       break;
     }
-    HeatArgs<RT> a = make_args<RT>(P, dt, lightsOn);
-    if (P->explicitScheme) {
-      const unsigned ge = (unsigned)(((int64_t)P->nx * P->ny * ((P->nz + kExplicitZ - 1) / kExplicitZ) + tpb - 1) / tpb);
-      hipLaunchKernelGGL(heat_explicit<RT>, dim3(ge), dim3(tpb), 0, P->stream, a);
-    } else {
-      launch_sweep_x<RT>(P, a, gx);
-      launch_sweep_yz<RT>(P, a, gy, gz);
-    }
+    enqueue_step<RT>(P, dt, lightsOn);
     MCM_HIP(hipGetLastError());
-    std::swap(P->T1.p, P->T2.p);
     ++P->stepsDone;
     if ((rc = record_sensors<RT>(P, P->stepsDone))) return rc;
   }
@@ -1366,6 +1493,136 @@ int mcm_heat_plan_advance(mcm_heat_plan* P, int64_t steps, double dt, int32_t li
   if (steps < 0) return fail(MCM_ERR_INVALID_ARGUMENT, "steps must be non-negative");
   MCM_HIP(hipSetDevice(P->device));
   return P->fp32 ? advance<float>(P, steps, dt, lightsOn) : advance<double>(P, steps, dt, lightsOn);
+}
+
+int mcm_heat_group_create(mcm_heat_plan* const* plans, int64_t nPlans, mcm_heat_group** out) {
+  if (!plans || !out || nPlans < 1) return fail(MCM_ERR_INVALID_ARGUMENT, "a group needs at least one plan");
+  *out = nullptr;
+  for (int64_t i = 0; i < nPlans; ++i) {
+    if (!plans[i]) return fail(MCM_ERR_INVALID_ARGUMENT, "plan %lld is null", (long long)i);
+    if (plans[i]->device != plans[0]->device) return fail(MCM_ERR_INVALID_ARGUMENT, "the plans of a group share one device");
+    for (int64_t j = 0; j < i; ++j)
+      if (plans[j] == plans[i]) return fail(MCM_ERR_INVALID_ARGUMENT, "plan %lld appears twice", (long long)i);
+  }
+  MCM_HIP(hipSetDevice(plans[0]->device));
+  std::unique_ptr<mcm_heat_group> G(new mcm_heat_group);
+  G->device = plans[0]->device;
+  G->plans.assign(plans, plans + nPlans);
+  G->events.assign((size_t)nPlans + 1, nullptr);
+  for (hipEvent_t& e : G->events) MCM_HIP(hipEventCreateWithFlags(&e, hipEventDisableTiming));
+  *out = G.release();
+  return MCM_OK;
+}
+
+int mcm_heat_group_advance(mcm_heat_group* G, int64_t steps, const double* dt, const int32_t* lightsOn, int64_t stepsPerGraph) {
+  if (!G || !dt || !lightsOn) return fail(MCM_ERR_INVALID_ARGUMENT, "null argument");
+  if (steps < 0 || stepsPerGraph < 0) return fail(MCM_ERR_INVALID_ARGUMENT, "steps must be non-negative");
+  const size_t n = G->plans.size();
+  mcm_heat_plan* P0 = G->plans[0];
+  MCM_HIP(hipSetDevice(P0->device));
+  for (size_t i = 0; i < n; ++i) {
+    int rc = ensure_sensor_capacity(G->plans[i], G->plans[i]->stepsDone + steps + 1);
+    if (rc) return rc;
+    if (G->plans[i]->nS && !G->plans[i]->stepDev.p) MCM_HIP(hipMalloc(&G->plans[i]->stepDev.p, sizeof(long long)));
+  }
+  const int64_t K = stepsPerGraph & ~(int64_t)1;
+  int64_t done = 0;
+  if (K >= 2 && steps >= K) {
+    mcm_heat_group::Graph* g = nullptr;
+    for (mcm_heat_group::Graph& c : G->graphs) {
+      bool same = c.steps == K;
+      for (size_t i = 0; same && i < n; ++i)
+        same = c.dt[i] == dt[i] && c.lightsOn[i] == (lightsOn[i] != 0) && c.T1[i] == G->plans[i]->T1.p && c.sensors[i] == G->plans[i]->sensors.p;
+      if (same) g = &c;
+    }
+    if (!g) {
+      // fork from P0's stream into every plan's stream, K steps per plan, join
+      MCM_HIP(hipStreamBeginCapture(P0->stream, hipStreamCaptureModeThreadLocal));
+      hipError_t err = hipEventRecord(G->events[0], P0->stream);
+      for (size_t i = 1; i < n && err == hipSuccess; ++i) err = hipStreamWaitEvent(G->plans[i]->stream, G->events[0], 0);
+      for (size_t i = 0; i < n && err == hipSuccess; ++i) {
+        mcm_heat_plan* P = G->plans[i];
+        if (P->fp32) enqueue_graph_steps<float>(P, K, dt[i], lightsOn[i] != 0);
+        else enqueue_graph_steps<double>(P, K, dt[i], lightsOn[i] != 0);
+        err = hipGetLastError();
+      }
+      for (size_t i = 1; i < n && err == hipSuccess; ++i) {
+        err = hipEventRecord(G->events[i + 1], G->plans[i]->stream);
+        if (err == hipSuccess) err = hipStreamWaitEvent(P0->stream, G->events[i + 1], 0);
+      }
+      hipGraph_t graph = nullptr;
+      const hipError_t endErr = hipStreamEndCapture(P0->stream, &graph);
+      if (err != hipSuccess || endErr != hipSuccess) {
+        if (graph) (void)hipGraphDestroy(graph);
+        return fail(MCM_ERR_DEVICE, "capturing the group's steps: %s", hipGetErrorString(err != hipSuccess ? err : endErr));
+      }
+      hipGraphExec_t exec = nullptr;
+      err = hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0);
+      (void)hipGraphDestroy(graph);
+      if (err != hipSuccess) return fail(MCM_ERR_DEVICE, "instantiating the group's step graph: %s", hipGetErrorString(err));
+      if (G->graphs.size() >= 4) {
+        (void)hipGraphExecDestroy(G->graphs.front().exec);
+        G->graphs.erase(G->graphs.begin());
+      }
+      mcm_heat_group::Graph c;
+      c.exec = exec;
+      c.steps = K;
+      for (size_t i = 0; i < n; ++i) {
+        c.dt.push_back(dt[i]);
+        c.lightsOn.push_back(lightsOn[i] != 0);
+        c.T1.push_back(G->plans[i]->T1.p);
+        c.sensors.push_back(G->plans[i]->sensors.p);
+      }
+      G->graphs.push_back(std::move(c));
+      g = &G->graphs.back();
+    }
+    // order the replays after every plan's earlier work; set the step counters
+    for (size_t i = 1; i < n; ++i) {
+      MCM_HIP(hipEventRecord(G->events[i + 1], G->plans[i]->stream));
+      MCM_HIP(hipStreamWaitEvent(P0->stream, G->events[i + 1], 0));
+    }
+    for (size_t i = 0; i < n; ++i)
+      if (G->plans[i]->nS)
+        hipLaunchKernelGGL(heat_step_counter, dim3(1), dim3(64), 0, P0->stream, (long long*)G->plans[i]->stepDev.p,
+                           (long long)G->plans[i]->stepsDone, 0);
+    for (; steps - done >= K; done += K) {
+      if (P0->interrupt && P0->interrupt()) {
+        P0->interrupted = true;
+        break;
+      }
+      MCM_HIP(hipGraphLaunch(g->exec, P0->stream));
+      for (size_t i = 0; i < n; ++i) G->plans[i]->stepsDone += K;
+    }
+    // later work on any plan's stream follows the replays
+    MCM_HIP(hipEventRecord(G->events[0], P0->stream));
+    for (size_t i = 1; i < n; ++i) MCM_HIP(hipStreamWaitEvent(G->plans[i]->stream, G->events[0], 0));
+    if (P0->interrupted) return MCM_OK;
+  }
+  for (size_t i = 0; i < n && done < steps; ++i) {
+    mcm_heat_plan* P = G->plans[i];
+    const int rc = P->fp32 ? advance<float>(P, steps - done, dt[i], lightsOn[i] != 0) : advance<double>(P, steps - done, dt[i], lightsOn[i] != 0);
+    if (rc) return rc;
+  }
+  return MCM_OK;
+}
+
+int mcm_heat_group_destroy(mcm_heat_group* G) {
+  if (!G) return MCM_OK;
+  // the plans may already be gone: wait for the device, not their streams
+  (void)hipSetDevice(G->device);
+  (void)hipDeviceSynchronize();
+  delete G;
+  return MCM_OK;
+}
+
+int mcm_heat_plan_set_graph_steps(mcm_heat_plan* P, int64_t steps) {
+  if (!P) return fail(MCM_ERR_INVALID_ARGUMENT, "null plan");
+  if (steps < 0) return fail(MCM_ERR_INVALID_ARGUMENT, "steps per graph must be non-negative");
+  MCM_HIP(hipSetDevice(P->device));
+  MCM_HIP(hipStreamSynchronize(P->stream));
+  P->drop_graphs();
+  P->graphSteps = steps;
+  return MCM_OK;
 }
 
 int mcm_heat_plan_download(mcm_heat_plan* P, mcm_heat_output* out) {
