@@ -231,6 +231,28 @@ typedef struct mcm_mc_item {
  * MCM_ERR_UNSUPPORTED; nItems * nM must not exceed 2048. */
 int mcm_mc_batch_run(mcm_mc_plan *plan, const mcm_mc_item *items, int64_t nItems, mcm_mc_output *outs);
 
+/* Work items on different grids (no reference counterpart): item i runs
+ * nPhotons packets of plans[plan] at wavelength iL with Philox key seed and
+ * stream id `stream`, and reproduces mcm_mc_plan_simulate of that plan when
+ * stream == iL.  The items run in one sequence of persistent launches per
+ * interface class (plans with refractive-index changes, plans without);
+ * each lane reads the grid, media and tally layout of its packet's plan.
+ * outs[i] is normalised like mcm_mc_batch_run's.  The plans share one
+ * device; deposition criteria and 3D sources are MCM_ERR_UNSUPPORTED, the
+ * items of one plan need equal refractive indices, nItems is 1..256 and
+ * nItems * (largest nM) must not exceed 2048.  Uses plans[0]'s stream,
+ * counters and packet records. */
+typedef struct mcm_mc_multi_item {
+  int64_t plan;        /* index into plans */
+  int64_t iL;          /* wavelength of that plan */
+  uint64_t nPhotons;
+  uint64_t seed;
+  uint32_t stream;
+  uint32_t reserved;   /* 0 */
+} mcm_mc_multi_item;
+int mcm_mc_multi_run(mcm_mc_plan *const *plans, int64_t nPlans, const mcm_mc_multi_item *items, int64_t nItems,
+                     mcm_mc_output *outs);
+
 /* Differentiable absorbed power (no reference counterpart).  J = power
  * absorbed inside a box of voxels per watt of source power (spectrum[iL]
  * scales it like the NFR normalisation, MCmatlablib.c:1195-1200), its

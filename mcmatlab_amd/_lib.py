@@ -25,6 +25,8 @@ SIGNATURES = {
     "mcm_mc_plan_destroy": (C.c_int, [C.c_void_p]),
     "mcm_mc_plan_autotune": (C.c_int, [C.c_void_p, C.c_int64, C.c_uint64, C.POINTER(C.c_double)]),
     "mcm_mc_batch_run": (C.c_int, [C.c_void_p, C.POINTER(abi.mcm_mc_item), C.c_int64, C.POINTER(abi.mcm_mc_output)]),
+    "mcm_mc_multi_run": (C.c_int, [C.POINTER(C.c_void_p), C.c_int64, C.POINTER(abi.mcm_mc_multi_item), C.c_int64,
+                                   C.POINTER(abi.mcm_mc_output)]),
     "mcm_mc_grad_run": (C.c_int, [C.c_void_p, C.POINTER(abi.mcm_mc_grad_input), C.POINTER(abi.mcm_mc_grad_output)]),
     "mcm_heat_run": (C.c_int, [C.POINTER(abi.mcm_heat_input), C.POINTER(abi.mcm_heat_output)]),
     "mcm_heat_plan_create": (C.c_int, [C.POINTER(abi.mcm_heat_input), C.POINTER(C.c_void_p)]),

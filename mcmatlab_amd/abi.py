@@ -280,6 +280,11 @@ class mcm_mc_item(C.Structure):
                 ("reserved", C.c_uint32)]
 
 
+class mcm_mc_multi_item(C.Structure):
+    _fields_ = [("plan", C.c_int64), ("iL", C.c_int64), ("nPhotons", C.c_uint64), ("seed", C.c_uint64),
+                ("stream", C.c_uint32), ("reserved", C.c_uint32)]
+
+
 class MCOutputs:
     """Caller-owned output buffers for one mcm_mc_output (for `nL`
     wavelengths; default: the input's)."""

@@ -88,6 +88,29 @@ def finiteElementHeatPropagator(T, Omega, P, precision="fp64", scheme="dgadi", s
 finiteElementHeatPropagator.last_state = (None, None)
 
 
+def multi_run(plans, items):
+    """mcm_mc_multi_run: items = [(plan, iL, nPhotons, seed, stream), ...]
+    over MCPlans of different grids, run in one sequence of persistent
+    launches per interface class; returns one output dict per item (one
+    wavelength each, normalised like MCmatlab)."""
+    plans = list(plans)
+    hs = (C.c_void_p * len(plans))(*[p.h.value for p in plans])
+    n = len(items)
+    arr = (abi.mcm_mc_multi_item * n)()
+    for k, (pi, iL, nph, seed, stream) in enumerate(items):
+        a = arr[k]
+        a.plan, a.iL, a.nPhotons, a.seed, a.stream, a.reserved = int(pi), int(iL), int(nph), int(seed), int(stream), 0
+    # (an out-of-range plan index is reported by the library)
+    outs = [abi.MCOutputs(plans[int(it[0]) if 0 <= int(it[0]) < len(plans) else 0]._inp, nL=1) for it in items]
+    structs = (abi.mcm_mc_output * n)(*[o.struct for o in outs])
+    check(lib().mcm_mc_multi_run(hs, len(plans), arr, n, structs))
+    res = []
+    for o, st in zip(outs, structs):
+        o.struct = st
+        res.append(o.result())
+    return res
+
+
 class HeatPlan:
     """Device-resident heat propagation (mcm_heat_plan_*): upload T, Omega
     and the model once, advance any schedule of (steps, dt, lightsOn) runs,

@@ -54,8 +54,24 @@ namespace mcm {
 // packet g - cum[i] of Philox stream (key[2i], key[2i+1], stream[i]), so an
 // item reproduces a plain launch of the same wavelength, seed and stream.
 // Media tables are per item ([nItems * nM]); tallies are per item slices.
+// Kernels take the kind of launch as a template argument: kPlain, kBatch
+// (items on the launch's grid) or kMulti (items on different grids: each
+// item names a model, whose grid, tables and tally layout it uses).
 constexpr int kBatchMaxItems = 256;
 constexpr int kBatchMaxMedia = 2048;  // nItems * nM entries staged in LDS
+constexpr int kPlain = 0, kBatch = 1, kMulti = 2;
+
+// What the transport reads of a model that differs between the models of a
+// multi-grid launch (copied into the lane's parameters at each fetch).
+struct McGeom {
+  int n[3];
+  float d[3], inv_d[3];
+  float tooOld;
+  int boundaryType, calcNFR, needTime, useLC, farFieldRes;
+  const uint8_t* M;
+  const float4* normalRI;
+  const float* cdfs;
+};
 
 struct McBatch {
   int nItems, nM;
@@ -70,6 +86,22 @@ struct McBatch {
   double* FF;                     // [nItems * nFF]
   double* NI[6];                  // [nItems * nNI[f]]
   unsigned long long* collected;  // [nItems]
+  // multi-grid launches
+  const int* model;               // [nItems] model of each item
+  const McParams* params;         // [nModels] (launch and escape tallies)
+  const McGeom* geom;             // [nModels]
+  const int64_t* off;             // [nItems * 9] offsets of the item's NFR, image, FF, NI[0..5] slices
+};
+
+// The parameters a lane uses: the launch's, or in a multi-grid launch the
+// lane's own copy, which each fetch points at the packet's model.
+template <int KIND>
+struct LaneParams {
+  __device__ static const McParams& get(const McParams& p, McParams&) { return p; }
+};
+template <>
+struct LaneParams<kMulti> {
+  __device__ static const McParams& get(const McParams&, McParams& q) { return q; }
 };
 
 // item owning global packet index g: the last i with cum[i] <= g
@@ -174,7 +206,7 @@ __device__ inline DeferSink make_sink<DeferSink>(const McParams& p, EscapeRecord
 // that fails 10^6 times flags the run as aborted and the remaining packets
 // are skipped (the transport kernel does not start any packet of an aborted
 // run).
-template <bool BATCH>
+template <int BATCH>
 __global__ void __launch_bounds__(256) mc_launch_packets(McParams p, McLaunch lp, McBatch bt, LaunchRecord* __restrict__ rec,
                                                          EscapeRecord* __restrict__ esc) {
   PacketRng R;
@@ -185,9 +217,10 @@ __global__ void __launch_bounds__(256) mc_launch_packets(McParams p, McLaunch lp
   for (uint64_t k = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; k < lp.nPackets; k += stride) {
     if (__hip_atomic_load(&lp.ctr->aborted, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)) break;
     const Medium* media = p.media;
-    if constexpr (BATCH) {
+    int it = 0;
+    if constexpr (BATCH != kPlain) {
       const unsigned long long g = lp.firstPacket + k;
-      const int it = batch_item(g, bt.cum, bt.nItems);
+      it = batch_item(g, bt.cum, bt.nItems);
       R.key0 = bt.key[2 * it];
       R.key1 = bt.key[2 * it + 1];
       R.stream = bt.stream[it];
@@ -197,7 +230,9 @@ __global__ void __launch_bounds__(256) mc_launch_packets(McParams p, McLaunch lp
       R.start(lp.firstPacket + k);
     }
     Photon P;
-    const bool ok = launch(P, p, media, R);
+    bool ok;
+    if constexpr (BATCH == kMulti) ok = launch(P, bt.params[bt.model[it]], media, R);
+    else ok = launch(P, p, media, R);
     LaunchRecord L;
     pack_launch(P, R, ok, L);
     rec[k] = L;
@@ -208,7 +243,7 @@ __global__ void __launch_bounds__(256) mc_launch_packets(McParams p, McLaunch lp
 
 // Light collector, far field and edge-flux tallies of the recorded escapes
 // (slots whose flags are zero belong to packets absorbed inside).
-template <bool BATCH>
+template <int BATCH>
 __global__ void __launch_bounds__(256) mc_tally_escapes(McParams p, McLaunch lp, McBatch bt, const EscapeRecord* __restrict__ rec) {
   DeviceSink sink(p);
   const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
@@ -219,7 +254,21 @@ __global__ void __launch_bounds__(256) mc_tally_escapes(McParams p, McLaunch lp,
     E.a = rec[k].a;
     E.b = rec[k].b;
     E.c = c;
-    if constexpr (BATCH) {
+    if constexpr (BATCH == kMulti) {
+      const int it = (int)c.z;
+      const int64_t* o = bt.off + 9 * (size_t)it;
+      DeviceSink s(p);
+      s.image = bt.image + o[1];
+      s.FF = bt.FF + o[2];
+      s.ni0 = bt.NI[0] + o[3];
+      s.ni1 = bt.NI[1] + o[4];
+      s.ni2 = bt.NI[2] + o[5];
+      s.ni3 = bt.NI[3] + o[6];
+      s.ni4 = bt.NI[4] + o[7];
+      s.ni5 = bt.NI[5] + o[8];
+      tally_escape(E, bt.params[bt.model[it]], s);
+      if (s.nCollected) atomicAdd(&bt.collected[it], s.nCollected);
+    } else if constexpr (BATCH == kBatch) {
       const int it = (int)c.z;
       DeviceSink s(p);
       s.image = bt.image + (size_t)it * bt.nImage;
@@ -239,15 +288,15 @@ __global__ void __launch_bounds__(256) mc_tally_escapes(McParams p, McLaunch lp,
   if (sink.nCollected) atomicAdd(&lp.ctr->collected, sink.nCollected);
 }
 
-template <bool FRESNEL, bool CRITERIA, bool BATCH>
+template <bool FRESNEL, bool CRITERIA, int BATCH>
 __global__ void __launch_bounds__(256) mc_transport(McParams p, McLaunch lp, McBatch bt, const LaunchRecord* __restrict__ launches,
                                                     EscapeRecord* __restrict__ escapes) {
-  static_assert(!(BATCH && CRITERIA), "batched launches run without deposition criteria");
-  constexpr int kMed = BATCH ? kBatchMaxMedia : 256;
+  static_assert(!(BATCH != kPlain && CRITERIA), "batched launches run without deposition criteria");
+  constexpr int kMed = BATCH != kPlain ? kBatchMaxMedia : 256;
   __shared__ Medium sMed[kMed];
   __shared__ uint8_t sCdf[kMed];
-  __shared__ unsigned long long sCum[BATCH ? kBatchMaxItems + 1 : 1];
-  if constexpr (BATCH) {
+  __shared__ unsigned long long sCum[BATCH != kPlain ? kBatchMaxItems + 1 : 1];
+  if constexpr (BATCH != kPlain) {
     for (int k = threadIdx.x; k < bt.nItems * bt.nM; k += blockDim.x) {
       sMed[k] = bt.media[k];
       sCdf[k] = bt.cdfIdx[k];
@@ -270,6 +319,9 @@ __global__ void __launch_bounds__(256) mc_transport(McParams p, McLaunch lp, McB
   const Medium* med = sMed;
   const uint8_t* cdf = sCdf;
   unsigned long long nLaunched = 0, nSub = 0, nScat = 0, nIf = 0;
+  McParams q;
+  if constexpr (BATCH == kMulti) q = p;
+  const McParams& pr = LaneParams<BATCH>::get(p, q);
 
   for (;;) {
     const int st = E.state;
@@ -291,17 +343,17 @@ __global__ void __launch_bounds__(256) mc_transport(McParams p, McLaunch lp, McB
     if (phase == ST_MOVE) {
       if (st == ST_MOVE) {
         ++nSub;
-        E.do_move(p, med, cdf, sink);
+        E.do_move(pr, med, cdf, sink);
       }
     } else if (phase == ST_SCATTER) {
       if (st == ST_SCATTER) {
         ++nScat;
-        E.do_scatter(p, med, cdf, sink);
+        E.do_scatter(pr, med, cdf, sink);
       }
     } else if (FRESNEL && phase == ST_FRESNEL) {
       if (st == ST_FRESNEL) {
         ++nIf;
-        E.do_fresnel(p, med, cdf, sink);
+        E.do_fresnel(pr, med, cdf, sink);
       }
     } else {
       // consecutive packet numbers for the waiting lanes, one atomic per wavefront
@@ -321,8 +373,8 @@ __global__ void __launch_bounds__(256) mc_transport(McParams p, McLaunch lp, McB
           bool ok;
           E.slot = (uint32_t)k;
           if constexpr (CRITERIA) {
-            ok = E.start(lp.firstPacket + k, p, med, cdf);
-          } else if constexpr (BATCH) {
+            ok = E.start(lp.firstPacket + k, pr, med, cdf);
+          } else if constexpr (BATCH != kPlain) {
             const unsigned long long g = lp.firstPacket + k;
             const int it = batch_item(g, sCum, bt.nItems);
             med = sMed + it * bt.nM;
@@ -330,13 +382,32 @@ __global__ void __launch_bounds__(256) mc_transport(McParams p, McLaunch lp, McB
             E.R.key0 = bt.key[2 * it];
             E.R.key1 = bt.key[2 * it + 1];
             E.R.stream = bt.stream[it];
-            sink.NFR = bt.NFR + (size_t)it * bt.L;
+            if constexpr (BATCH == kMulti) {
+              const McGeom& G = bt.geom[bt.model[it]];
+              for (int a = 0; a < 3; ++a) {
+                q.n[a] = G.n[a];
+                q.d[a] = G.d[a];
+                q.inv_d[a] = G.inv_d[a];
+              }
+              q.tooOld = G.tooOld;
+              q.boundaryType = G.boundaryType;
+              q.calcNFR = G.calcNFR;
+              q.needTime = G.needTime;
+              q.useLC = G.useLC;
+              q.farFieldRes = G.farFieldRes;
+              q.M = G.M;
+              q.normalRI = G.normalRI;
+              q.cdfs = G.cdfs;
+              sink.NFR = bt.NFR + bt.off[9 * (size_t)it];
+            } else {
+              sink.NFR = bt.NFR + (size_t)it * bt.L;
+            }
             sink.item = it;
             const LaunchRecord L = launches[k];
-            ok = E.start_from(g - sCum[it], L, p, med, cdf);
+            ok = E.start_from(g - sCum[it], L, pr, med, cdf);
           } else {
             const LaunchRecord L = launches[k];
-            ok = E.start_from(lp.firstPacket + k, L, p, med, cdf);
+            ok = E.start_from(lp.firstPacket + k, L, pr, med, cdf);
           }
           if (ok) {
             ++nLaunched;
@@ -563,7 +634,7 @@ int ensure_records(mcm_mc_plan* pl, uint64_t n) {
 // transport launches and tallies inline.  `bt` (batched launches) maps the
 // global packet indices first .. first+n-1 to work items.
 int launch_transport(mcm_mc_plan* pl, uint64_t first, uint64_t n, uint64_t seed, uint32_t stream, hipStream_t s,
-                     const McBatch* bt = nullptr) {
+                     const McBatch* bt = nullptr, int kind = kPlain, bool fresnel = false, bool tallies = false) {
   McLaunch lp;
   lp.firstPacket = first;
   lp.nPackets = n;
@@ -572,8 +643,8 @@ int launch_transport(mcm_mc_plan* pl, uint64_t first, uint64_t n, uint64_t seed,
   lp.stream = stream;
   lp.moveFrac = pl->moveFrac;
   lp.ctr = pl->counters.as<McCounters>();
-  const bool fr = pl->T.fresnelPossible;
-  const bool cr = pl->prm.criteriaActive != 0;
+  const bool fr = kind == kMulti ? fresnel : pl->T.fresnelPossible;
+  const bool cr = kind == kPlain && pl->prm.criteriaActive != 0;
   McBatch none;
   std::memset(&none, 0, sizeof none);
   const McBatch& B = bt ? *bt : none;
@@ -581,8 +652,8 @@ int launch_transport(mcm_mc_plan* pl, uint64_t first, uint64_t n, uint64_t seed,
   MCM_HIP(hipMemsetAsync(pl->counters.p, 0, sizeof(unsigned long long), s));  // nextPacket
   if (cr) {
     const dim3 g(pl->blocks);
-    if (fr) hipLaunchKernelGGL((mc_transport<true, true, false>), g, b, 0, s, pl->prm, lp, B, nullptr, nullptr);
-    else hipLaunchKernelGGL((mc_transport<false, true, false>), g, b, 0, s, pl->prm, lp, B, nullptr, nullptr);
+    if (fr) hipLaunchKernelGGL((mc_transport<true, true, kPlain>), g, b, 0, s, pl->prm, lp, B, nullptr, nullptr);
+    else hipLaunchKernelGGL((mc_transport<false, true, kPlain>), g, b, 0, s, pl->prm, lp, B, nullptr, nullptr);
     MCM_HIP(hipGetLastError());
     return MCM_OK;
   }
@@ -593,32 +664,86 @@ int launch_transport(mcm_mc_plan* pl, uint64_t first, uint64_t n, uint64_t seed,
   const McTables& T = pl->T;
   bool anyNI = false;
   for (int f = 0; f < 6; ++f) anyNI = anyNI || T.nNI[f] > 0;
-  const bool tally = T.base.useLC || T.base.farFieldRes || anyNI;
-  if (bt) {
+  const bool tally = kind == kMulti ? tallies : (T.base.useLC || T.base.farFieldRes || anyNI);
+  if (kind == kMulti) {
+    const dim3 g(pl->blocksMulti);
+    hipLaunchKernelGGL(mc_launch_packets<kMulti>, g, b, 0, s, pl->prm, lp, B, L, E);
+    if (fr) hipLaunchKernelGGL((mc_transport<true, false, kMulti>), g, b, 0, s, pl->prm, lp, B, L, E);
+    else hipLaunchKernelGGL((mc_transport<false, false, kMulti>), g, b, 0, s, pl->prm, lp, B, L, E);
+    if (tally) hipLaunchKernelGGL(mc_tally_escapes<kMulti>, g, b, 0, s, pl->prm, lp, B, E);
+  } else if (bt) {
     const dim3 g(pl->blocksBatch);
-    hipLaunchKernelGGL(mc_launch_packets<true>, g, b, 0, s, pl->prm, lp, B, L, E);
-    if (fr) hipLaunchKernelGGL((mc_transport<true, false, true>), g, b, 0, s, pl->prm, lp, B, L, E);
-    else hipLaunchKernelGGL((mc_transport<false, false, true>), g, b, 0, s, pl->prm, lp, B, L, E);
-    if (tally) hipLaunchKernelGGL(mc_tally_escapes<true>, g, b, 0, s, pl->prm, lp, B, E);
+    hipLaunchKernelGGL(mc_launch_packets<kBatch>, g, b, 0, s, pl->prm, lp, B, L, E);
+    if (fr) hipLaunchKernelGGL((mc_transport<true, false, kBatch>), g, b, 0, s, pl->prm, lp, B, L, E);
+    else hipLaunchKernelGGL((mc_transport<false, false, kBatch>), g, b, 0, s, pl->prm, lp, B, L, E);
+    if (tally) hipLaunchKernelGGL(mc_tally_escapes<kBatch>, g, b, 0, s, pl->prm, lp, B, E);
   } else {
     const dim3 g(pl->blocks);
-    hipLaunchKernelGGL(mc_launch_packets<false>, g, b, 0, s, pl->prm, lp, B, L, E);
-    if (fr) hipLaunchKernelGGL((mc_transport<true, false, false>), g, b, 0, s, pl->prm, lp, B, L, E);
-    else hipLaunchKernelGGL((mc_transport<false, false, false>), g, b, 0, s, pl->prm, lp, B, L, E);
-    if (tally) hipLaunchKernelGGL(mc_tally_escapes<false>, g, b, 0, s, pl->prm, lp, B, E);
+    hipLaunchKernelGGL(mc_launch_packets<kPlain>, g, b, 0, s, pl->prm, lp, B, L, E);
+    if (fr) hipLaunchKernelGGL((mc_transport<true, false, kPlain>), g, b, 0, s, pl->prm, lp, B, L, E);
+    else hipLaunchKernelGGL((mc_transport<false, false, kPlain>), g, b, 0, s, pl->prm, lp, B, L, E);
+    if (tally) hipLaunchKernelGGL(mc_tally_escapes<kPlain>, g, b, 0, s, pl->prm, lp, B, E);
   }
   MCM_HIP(hipGetLastError());
   return MCM_OK;
 }
 
-int occupancy_blocks(int device, bool fr, bool cr, bool batch = false) {
+// Normalised outputs of one work item from its FP64 tally slices
+// (normalizeDepositionAndResetO, MCmatlablib.c:1186-1250); null slices are
+// skipped.
+int finish_item(mcm_mc_plan* pl, int64_t iL, uint64_t nPhotons, const double* nfr, const double* img, const double* ff,
+                const double* const ni[6], unsigned long long collected, double minutes, int64_t lanes, mcm_mc_output& o) {
+  const McTables& T = pl->T;
+  hipStream_t s = pl->stream;
+  std::vector<double> h;
+  auto slice = [&](const double* src, int64_t n, float* dst, double div) -> int {
+    if (n <= 0 || !dst || !src) return MCM_OK;
+    h.resize((size_t)n);
+    MCM_HIP(hipMemcpy(h.data(), src, (size_t)n * sizeof(double), hipMemcpyDeviceToHost));
+    for (int64_t k = 0; k < n; ++k) dst[k] = (float)(h[k] / div);
+    return MCM_OK;
+  };
+  int rc;
+  double normfactor = (double)nPhotons / T.spectrum[(size_t)iL];
+  if (T.base.beamType == 2 && (T.base.boundaryType == 0 || T.base.boundaryType == 2)) normfactor /= kKillRange * kKillRange;
+  if (T.calcNFR && o.NFR && nfr && nPhotons) {
+    if (!pl->nfrOut.p) MCM_HIP(hipMalloc(&pl->nfrOut.p, (size_t)T.L * sizeof(float)));
+    const int tpb = 256;
+    const int64_t nb = (T.L + tpb - 1) / tpb;
+    hipLaunchKernelGGL(mc_normalize_nfr, dim3((unsigned)nb), dim3(tpb), 0, s, const_cast<double*>(nfr), pl->M.as<uint8_t>(),
+                       pl->muaNorm.as<double>() + (size_t)iL * T.nM, T.V * normfactor, pl->nfrOut.as<float>(), T.L);
+    MCM_HIP(hipGetLastError());
+    MCM_HIP(hipMemcpyAsync(o.NFR, pl->nfrOut.p, (size_t)T.L * sizeof(float), hipMemcpyDeviceToHost, s));
+    MCM_HIP(hipStreamSynchronize(s));
+  }
+  if (T.base.useLC) {
+    const int64_t LLC = (int64_t)T.lcRes0 * T.lcRes0;
+    const double div = LLC > 1 ? (double)T.base.lcFSorNA * (double)T.base.lcFSorNA / LLC * normfactor : normfactor;
+    if ((rc = slice(img, T.nImage, o.image, div))) return rc;
+  }
+  if (T.nFF && (rc = slice(ff, T.nFF, o.farField, normfactor))) return rc;
+  float* nio[6] = {o.NI_xpos, o.NI_xneg, o.NI_ypos, o.NI_yneg, o.NI_zpos, o.NI_zneg};
+  const double area[6] = {T.dA[0], T.dA[0], T.dA[1], T.dA[1], T.dA[2], T.dA[2]};
+  for (int f = 0; f < 6; ++f)
+    if (T.nNI[f] && (rc = slice(ni[f], T.nNI[f], nio[f], area[f] * normfactor))) return rc;
+  o.nPhotons = (double)nPhotons;
+  o.nPhotonsCollected = (double)collected;
+  o.nThreads = (double)lanes;
+  o.simulationTime = minutes;
+  o.examplePaths = nullptr;
+  o.examplePathsElems = 0;
+  return MCM_OK;
+}
+
+int occupancy_blocks(int device, bool fr, bool cr, int kind = kPlain) {
   int per = 0;
   const void* k;
-  if (batch) k = fr ? (const void*)mc_transport<true, false, true> : (const void*)mc_transport<false, false, true>;
-  else if (fr && cr) k = (const void*)mc_transport<true, true, false>;
-  else if (fr) k = (const void*)mc_transport<true, false, false>;
-  else if (cr) k = (const void*)mc_transport<false, true, false>;
-  else k = (const void*)mc_transport<false, false, false>;
+  if (kind == kMulti) k = fr ? (const void*)mc_transport<true, false, kMulti> : (const void*)mc_transport<false, false, kMulti>;
+  else if (kind == kBatch) k = fr ? (const void*)mc_transport<true, false, kBatch> : (const void*)mc_transport<false, false, kBatch>;
+  else if (fr && cr) k = (const void*)mc_transport<true, true, kPlain>;
+  else if (fr) k = (const void*)mc_transport<true, false, kPlain>;
+  else if (cr) k = (const void*)mc_transport<false, true, kPlain>;
+  else k = (const void*)mc_transport<false, false, kPlain>;
   const hipError_t e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&per, k, 256, 0);
   if (e != hipSuccess || per < 1) per = 4;
   return per * cu_count(device);
@@ -698,9 +823,10 @@ int mcm_mc_plan_create(const mcm_mc_input* in, mcm_mc_plan** out) {
   for (int f = 0; f < 6; ++f) p.NI[f] = pl->NI[f].as<double>();
   pl->threads = 256;
   pl->blocks = occupancy_blocks(pl->device, T.fresnelPossible, p.criteriaActive != 0);
-  pl->blocksBatch = occupancy_blocks(pl->device, T.fresnelPossible, false, true);
+  pl->blocksBatch = occupancy_blocks(pl->device, T.fresnelPossible, false, kBatch);
+  pl->blocksMulti = std::min(occupancy_blocks(pl->device, true, false, kMulti), occupancy_blocks(pl->device, false, false, kMulti));
   if (const char* e = std::getenv("MCM_MC_MOVE_FRAC")) pl->moveFrac = (float)std::atof(e);
-  if (const char* e = std::getenv("MCM_MC_BLOCKS")) pl->blocks = pl->blocksBatch = std::max(1, std::atoi(e));
+  if (const char* e = std::getenv("MCM_MC_BLOCKS")) pl->blocks = pl->blocksBatch = pl->blocksMulti = std::max(1, std::atoi(e));
   if ((rc = select_wavelength(pl.get(), 0))) return rc;
   MCM_HIP(hipStreamSynchronize(s));
   *out = pl.release();
@@ -940,46 +1066,176 @@ int mcm_mc_batch_run(mcm_mc_plan* pl, const mcm_mc_item* items, int64_t nItems, 
   if (c.aborted) return fail(MCM_ERR_INVALID_ARGUMENT, "all packets launch outside the simulation cuboid");
   std::vector<unsigned long long> coll((size_t)nI);
   MCM_HIP(hipMemcpy(coll.data(), dColl.p, coll.size() * sizeof(unsigned long long), hipMemcpyDeviceToHost));
-  // per-item normalisation (normalizeDepositionAndResetO, MCmatlablib.c:1186-1250)
-  std::vector<double> h;
-  auto slice = [&](const DevBuf& b, int64_t off, int64_t n, float* dst, double div) -> int {
-    if (n <= 0 || !dst) return MCM_OK;
-    h.resize((size_t)n);
-    MCM_HIP(hipMemcpy(h.data(), b.as<double>() + off, (size_t)n * sizeof(double), hipMemcpyDeviceToHost));
-    for (int64_t k = 0; k < n; ++k) dst[k] = (float)(h[k] / div);
-    return MCM_OK;
-  };
   for (int i = 0; i < nI; ++i) {
-    const int64_t iL = items[i].iL;
-    mcm_mc_output& o = outs[i];
-    double normfactor = (double)items[i].nPhotons / T.spectrum[(size_t)iL];
-    if (T.base.beamType == 2 && (T.base.boundaryType == 0 || T.base.boundaryType == 2)) normfactor /= kKillRange * kKillRange;
-    if (T.calcNFR && o.NFR && items[i].nPhotons) {
-      if (!pl->nfrOut.p) MCM_HIP(hipMalloc(&pl->nfrOut.p, (size_t)T.L * sizeof(float)));
-      const int tpb = 256;
-      const int64_t nb = (T.L + tpb - 1) / tpb;
-      hipLaunchKernelGGL(mc_normalize_nfr, dim3((unsigned)nb), dim3(tpb), 0, s, dNFR.as<double>() + (size_t)i * T.L, pl->M.as<uint8_t>(),
-                         pl->muaNorm.as<double>() + (size_t)iL * T.nM, T.V * normfactor, pl->nfrOut.as<float>(), T.L);
-      MCM_HIP(hipGetLastError());
-      MCM_HIP(hipMemcpyAsync(o.NFR, pl->nfrOut.p, (size_t)T.L * sizeof(float), hipMemcpyDeviceToHost, s));
-      MCM_HIP(hipStreamSynchronize(s));
+    const double* ni[6];
+    for (int f = 0; f < 6; ++f) ni[f] = dNI[f].as<double>() + (size_t)i * T.nNI[f];
+    if ((rc = finish_item(pl, items[i].iL, items[i].nPhotons, T.calcNFR ? dNFR.as<double>() + (size_t)i * T.L : nullptr,
+                          dImg.as<double>() + (size_t)i * T.nImage, dFF.as<double>() + (size_t)i * T.nFF, ni, coll[i], minutes,
+                          (int64_t)pl->blocksBatch * pl->threads, outs[i])))
+      return rc;
+  }
+  return MCM_OK;
+}
+
+int mcm_mc_multi_run(mcm_mc_plan* const* plans, int64_t nPlans, const mcm_mc_multi_item* items, int64_t nItems, mcm_mc_output* outs) {
+  if (!plans || !items || !outs || nPlans < 1) return fail(MCM_ERR_INVALID_ARGUMENT, "null argument");
+  if (nItems < 1 || nItems > kBatchMaxItems) return fail(MCM_ERR_INVALID_ARGUMENT, "nItems must be 1..%d", kBatchMaxItems);
+  for (int64_t k = 0; k < nPlans; ++k) {
+    if (!plans[k]) return fail(MCM_ERR_INVALID_ARGUMENT, "plan %lld is null", (long long)k);
+    if (plans[k]->device != plans[0]->device) return fail(MCM_ERR_INVALID_ARGUMENT, "the plans of a multi-grid launch share one device");
+  }
+  const int nI = (int)nItems;
+  int nMmax = 1;
+  std::vector<int64_t> firstL((size_t)nPlans, -1);
+  for (int i = 0; i < nI; ++i) {
+    const mcm_mc_multi_item& it = items[i];
+    if (it.plan < 0 || it.plan >= nPlans) return fail(MCM_ERR_INVALID_ARGUMENT, "item %d: plan index out of range", i);
+    mcm_mc_plan* pl = plans[it.plan];
+    const McTables& T = pl->T;
+    if (it.iL < 0 || it.iL >= T.nL) return fail(MCM_ERR_INVALID_ARGUMENT, "item %d: wavelength out of range", i);
+    if (pl->prm.criteriaActive) return fail(MCM_ERR_UNSUPPORTED, "multi-grid launches run without deposition criteria");
+    if (pl->sourceDistribution) return fail(MCM_ERR_UNSUPPORTED, "multi-grid launches need beam sources");
+    int64_t& l0 = firstL[(size_t)it.plan];
+    if (l0 < 0) l0 = it.iL;
+    if (T.fresnelPossible)
+      for (int m = 0; m < T.nM; ++m)
+        if (T.media[(size_t)it.iL * 256 + m].ri != T.media[(size_t)l0 * 256 + m].ri)
+          return fail(MCM_ERR_UNSUPPORTED, "items of one plan need the same refractive indices");
+    nMmax = std::max(nMmax, T.nM);
+  }
+  if (nI * nMmax > kBatchMaxMedia) return fail(MCM_ERR_UNSUPPORTED, "nItems * (largest nM) exceeds %d", kBatchMaxMedia);
+  mcm_mc_plan* pl0 = plans[0];
+  MCM_HIP(hipSetDevice(pl0->device));
+  int rc;
+  // every used plan's tables at its items' refractive indices
+  for (int64_t k = 0; k < nPlans; ++k)
+    if (firstL[(size_t)k] >= 0) {
+      if ((rc = select_wavelength(plans[k], firstL[(size_t)k]))) return rc;
+      MCM_HIP(hipStreamSynchronize(plans[k]->stream));
     }
-    if (T.base.useLC) {
-      const int64_t LLC = (int64_t)T.lcRes0 * T.lcRes0;
-      const double div = LLC > 1 ? (double)T.base.lcFSorNA * (double)T.base.lcFSorNA / LLC * normfactor : normfactor;
-      if ((rc = slice(dImg, (int64_t)i * T.nImage, T.nImage, o.image, div))) return rc;
+  hipStream_t s = pl0->stream;
+  // tally slices of every item in one buffer per kind
+  std::vector<int64_t> off((size_t)nI * 9);
+  int64_t tot[9] = {0};
+  bool tallies = false;
+  for (int i = 0; i < nI; ++i) {
+    const McTables& T = plans[items[i].plan]->T;
+    const int64_t sz[9] = {T.calcNFR ? T.L : 0, T.nImage, T.nFF, T.nNI[0], T.nNI[1], T.nNI[2], T.nNI[3], T.nNI[4], T.nNI[5]};
+    for (int k = 0; k < 9; ++k) {
+      off[(size_t)i * 9 + k] = tot[k];
+      tot[k] += sz[k];
+      if (k > 0 && sz[k] > 0) tallies = true;
     }
-    if (T.nFF && (rc = slice(dFF, (int64_t)i * T.nFF, T.nFF, o.farField, normfactor))) return rc;
-    float* ni[6] = {o.NI_xpos, o.NI_xneg, o.NI_ypos, o.NI_yneg, o.NI_zpos, o.NI_zneg};
-    const double area[6] = {T.dA[0], T.dA[0], T.dA[1], T.dA[1], T.dA[2], T.dA[2]};
-    for (int f = 0; f < 6; ++f)
-      if (T.nNI[f] && (rc = slice(dNI[f], (int64_t)i * T.nNI[f], T.nNI[f], ni[f], area[f] * normfactor))) return rc;
-    o.nPhotons = (double)items[i].nPhotons;
-    o.nPhotonsCollected = (double)coll[i];
-    o.nThreads = (double)pl->blocksBatch * pl->threads;
-    o.simulationTime = minutes;
-    o.examplePaths = nullptr;
-    o.examplePathsElems = 0;
+    tallies = tallies || T.base.useLC || T.base.farFieldRes;
+  }
+  DevBuf dTal[9], dColl, dOff, dParams, dGeom;
+  for (int k = 0; k < 9; ++k)
+    if ((rc = zeros(dTal[k], (size_t)std::max<int64_t>(tot[k], 1) * sizeof(double), s))) return rc;
+  if ((rc = zeros(dColl, (size_t)nI * sizeof(unsigned long long), s)) || (rc = upload(dOff, off.data(), off.size(), s))) return rc;
+  std::vector<McParams> params((size_t)nPlans);
+  std::vector<McGeom> geom((size_t)nPlans);
+  for (int64_t k = 0; k < nPlans; ++k) {
+    const McParams& p = plans[k]->prm;
+    params[(size_t)k] = p;
+    McGeom& g = geom[(size_t)k];
+    for (int a = 0; a < 3; ++a) {
+      g.n[a] = p.n[a];
+      g.d[a] = p.d[a];
+      g.inv_d[a] = p.inv_d[a];
+    }
+    g.tooOld = p.tooOld;
+    g.boundaryType = p.boundaryType;
+    g.calcNFR = p.calcNFR;
+    g.needTime = p.needTime;
+    g.useLC = p.useLC;
+    g.farFieldRes = p.farFieldRes;
+    g.M = p.M;
+    g.normalRI = p.normalRI;
+    g.cdfs = p.cdfs;
+  }
+  if ((rc = upload(dParams, params.data(), params.size(), s)) || (rc = upload(dGeom, geom.data(), geom.size(), s))) return rc;
+  MCM_HIP(hipMemsetAsync(pl0->counters.p, 0, sizeof(McCounters), s));
+  const auto t0 = std::chrono::steady_clock::now();
+  // one sequence of launches per interface class (Fresnel or matched), so
+  // that every item runs the kernel its plan's own launches use
+  for (int cls = 0; cls < 2; ++cls) {
+    std::vector<int> idx;
+    for (int i = 0; i < nI; ++i)
+      if (plans[items[i].plan]->T.fresnelPossible == (cls == 0)) idx.push_back(i);
+    if (idx.empty()) continue;
+    const int n = (int)idx.size();
+    std::vector<unsigned long long> cum((size_t)n + 1, 0);
+    std::vector<uint32_t> key(2 * (size_t)n), strm((size_t)n);
+    std::vector<int> model((size_t)n);
+    std::vector<int64_t> offc((size_t)n * 9);
+    std::vector<Medium> med((size_t)n * nMmax);
+    std::vector<uint8_t> cdf((size_t)n * nMmax, 0);
+    for (int j = 0; j < n; ++j) {
+      const mcm_mc_multi_item& it = items[idx[(size_t)j]];
+      const McTables& T = plans[it.plan]->T;
+      cum[(size_t)j + 1] = cum[(size_t)j] + it.nPhotons;
+      key[2 * (size_t)j] = (uint32_t)it.seed;
+      key[2 * (size_t)j + 1] = (uint32_t)(it.seed >> 32);
+      strm[(size_t)j] = it.stream;
+      model[(size_t)j] = (int)it.plan;
+      for (int k = 0; k < 9; ++k) offc[(size_t)j * 9 + k] = off[(size_t)idx[(size_t)j] * 9 + k];
+      for (int m = 0; m < T.nM; ++m) {
+        med[(size_t)j * nMmax + m] = T.media[(size_t)it.iL * 256 + m];
+        cdf[(size_t)j * nMmax + m] = T.cdfIdx[(size_t)it.iL * 256 + m];
+      }
+    }
+    DevBuf dCum, dKey, dStrm, dModel, dOffc, dMed, dCdf, dCollc;
+    if ((rc = upload(dCum, cum.data(), cum.size(), s)) || (rc = upload(dKey, key.data(), key.size(), s)) ||
+        (rc = upload(dStrm, strm.data(), strm.size(), s)) || (rc = upload(dModel, model.data(), model.size(), s)) ||
+        (rc = upload(dOffc, offc.data(), offc.size(), s)) || (rc = upload(dMed, med.data(), med.size(), s)) ||
+        (rc = upload(dCdf, cdf.data(), cdf.size(), s)) || (rc = zeros(dCollc, (size_t)n * sizeof(unsigned long long), s)))
+      return rc;
+    McBatch B;
+    std::memset(&B, 0, sizeof B);
+    B.nItems = n;
+    B.nM = nMmax;
+    B.cum = dCum.as<unsigned long long>();
+    B.key = dKey.as<uint32_t>();
+    B.stream = dStrm.as<uint32_t>();
+    B.media = dMed.as<Medium>();
+    B.cdfIdx = dCdf.as<uint8_t>();
+    B.NFR = dTal[0].as<double>();
+    B.image = dTal[1].as<double>();
+    B.FF = dTal[2].as<double>();
+    for (int f = 0; f < 6; ++f) B.NI[f] = dTal[3 + f].as<double>();
+    B.collected = dCollc.as<unsigned long long>();
+    B.model = dModel.as<int>();
+    B.params = dParams.as<McParams>();
+    B.geom = dGeom.as<McGeom>();
+    B.off = dOffc.as<int64_t>();
+    const unsigned long long total = cum[(size_t)n];
+    for (unsigned long long done = 0; done < total;) {
+      const uint64_t m = std::min<unsigned long long>(total - done, kMaxLaunchPackets);
+      if ((rc = launch_transport(pl0, done, m, 0, 0, s, &B, kMulti, cls == 0, tallies))) return rc;
+      done += m;
+    }
+    std::vector<unsigned long long> cc((size_t)n);
+    MCM_HIP(hipMemcpyAsync(cc.data(), dCollc.p, cc.size() * sizeof(unsigned long long), hipMemcpyDeviceToHost, s));
+    MCM_HIP(hipStreamSynchronize(s));
+    for (int j = 0; j < n; ++j) outs[idx[(size_t)j]].nPhotonsCollected = (double)cc[(size_t)j];
+  }
+  MCM_HIP(hipStreamSynchronize(s));
+  const double minutes = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count() / 60.0;
+  McCounters c;
+  MCM_HIP(hipMemcpy(&c, pl0->counters.p, sizeof c, hipMemcpyDeviceToHost));
+  pl0->stats = c;
+  MCM_HIP(hipMemsetAsync(pl0->counters.p, 0, sizeof(McCounters), s));
+  if (c.aborted) return fail(MCM_ERR_INVALID_ARGUMENT, "all packets launch outside the simulation cuboid");
+  for (int i = 0; i < nI; ++i) {
+    mcm_mc_plan* pl = plans[items[i].plan];
+    const int64_t* o = &off[(size_t)i * 9];
+    const double* ni[6];
+    for (int f = 0; f < 6; ++f) ni[f] = dTal[3 + f].as<double>() + o[3 + f];
+    const unsigned long long coll = (unsigned long long)outs[i].nPhotonsCollected;
+    if ((rc = finish_item(pl, items[i].iL, items[i].nPhotons, pl->T.calcNFR ? dTal[0].as<double>() + o[0] : nullptr,
+                          dTal[1].as<double>() + o[1], dTal[2].as<double>() + o[2], ni, coll, minutes,
+                          (int64_t)pl0->blocksMulti * pl0->threads, outs[i])))
+      return rc;
   }
   return MCM_OK;
 }

@@ -48,7 +48,7 @@ struct mcm_mc_plan {
   hipStream_t stream = nullptr;
   mcm::McTables T;
   mcm::McParams prm;
-  int blocks = 0, blocksBatch = 0, threads = 256;
+  int blocks = 0, blocksBatch = 0, blocksMulti = 0, threads = 256;
   float moveFrac = 0.5f;
   DevBuf M, normals, media, cdfIdx, cdfs, fpid1, aid1, fpid2, aid2, S, muaNorm;
   DevBuf NFR, image, FF, NI[6], counters, nfrOut;

@@ -37,7 +37,7 @@ def test_library_exports_every_header_symbol():
     assert set(header_functions()) == set(_lib.SIGNATURES), "ctypes signatures out of sync with the header"
 
 
-STRUCTS = ["mcm_beam_distr", "mcm_mc_input", "mcm_mc_output", "mcm_mc_sizes", "mcm_mc_item", "mcm_mc_grad_input",
+STRUCTS = ["mcm_beam_distr", "mcm_mc_input", "mcm_mc_output", "mcm_mc_sizes", "mcm_mc_item", "mcm_mc_multi_item", "mcm_mc_grad_input",
            "mcm_mc_grad_output", "mcm_heat_input", "mcm_heat_output", "mcm_heat_segment", "mcm_heat_adjoint_output"]
 
 
