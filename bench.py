@@ -265,6 +265,80 @@ def bench_adjoint(n=128, steps=3000, every=100):
             "dJ_dQ_max": float(abs(r["dJ_dQ"]).max())}
 
 
+def bench_sweep(n_models=32, seed=3, heat_steps=1000, every=100):
+    """Batched forward + backward sweep over independent models on uneven
+    grids (multi_grid_sweep: sides 64..247 from seed 3, cyclic x/y, 1e6-1e7
+    packets each): forward MC of every model as one multi-grid launch
+    sequence (and, for comparison, as plain launches one model after
+    another), the absorbed-power gradient of each model, 1000 heat steps of
+    every model as one plan group (and one plan after another), and each
+    model's heat adjoint.  Stages print progress to stderr."""
+    from mcmatlab_amd import api
+    from mcmatlab_amd.workloads import multi_grid_sweep
+
+    def log(msg):
+        print(f"[sweep] {msg}", file=sys.stderr, flush=True)
+
+    t = time.perf_counter()
+    W = multi_grid_sweep(n_models=n_models, seed=seed)
+    log(f"models built in {time.perf_counter() - t:.1f}s")
+    plans = [api.MCPlan(w["model"]) for w in W]
+    hip = Hip()
+    packets = sum(w["nPhotons"] for w in W)
+    voxels = sum(w["side"] ** 3 for w in W)
+    api.multi_run(plans, [(k, 0, 20000, 7, 0) for k in range(len(W))])  # warm-up
+    items = [(k, 0, w["nPhotons"], 11 + k, 0) for k, w in enumerate(W)]
+    hip.sync()
+    t = time.perf_counter()
+    outs = api.multi_run(plans, items)
+    t_multi = time.perf_counter() - t
+    log(f"multi-grid MC {t_multi:.2f}s")
+    t = time.perf_counter()
+    for p, (k, iL, n, sd, st) in zip(plans, items):
+        p.simulate(n, seed=sd, iL=iL)
+        p.normalize(iL)
+    t_plain = time.perf_counter() - t
+    log(f"plain MC {t_plain:.2f}s")
+    t = time.perf_counter()
+    for p, w in zip(plans, W):
+        p.absorbed_power(w["box"], w["nPhotons"], seed=5)
+    t_grad = time.perf_counter() - t
+    log(f"gradients {t_grad:.2f}s")
+    for p in plans:
+        p.close()
+    cases = [w["heat"](o["NFR"][..., 0]) for w, o in zip(W, outs)]
+    hplans = [api.HeatPlan(T0, Om0, P) for T0, Om0, P in cases]
+    dts = [P["dt"] for _, _, P in cases]
+    hip.sync()
+    t = time.perf_counter()
+    with api.HeatGroup(hplans, steps_per_graph=32) as g:
+        g.advance(heat_steps, dts, [1] * len(hplans))
+        hip.sync()
+    t_group = time.perf_counter() - t
+    log(f"heat group {t_group:.2f}s")
+    t = time.perf_counter()
+    for h, d in zip(hplans, dts):
+        h.advance(heat_steps, d, 1)
+        hip.sync()
+    t_serial = time.perf_counter() - t
+    log(f"heat one plan after another {t_serial:.2f}s")
+    for h in hplans:
+        h.close()
+    t = time.perf_counter()
+    J = 0.0
+    for T0, Om0, P in cases:
+        r = api.heat_adjoint(T0, Om0, P, segments=[(heat_steps, P["dt"], True)], checkpointEvery=every)
+        J += r["J"]
+    t_adj = time.perf_counter() - t
+    log(f"adjoints {t_adj:.2f}s")
+    total = t_multi + t_grad + t_group + t_adj
+    return {"metric": f"seconds, {n_models}-grid sweep (sides 64..256): forward MC + mua/mus gradients + {heat_steps}-step heat + adjoint",
+            "value": total, "unit": "s", "higher_is_better": False, "models": n_models, "voxels": voxels, "packets": packets,
+            "mc_multi_s": t_multi, "mc_plain_s": t_plain, "mc_multi_photons_per_s": packets / t_multi, "grad_s": t_grad,
+            "grad_packets_per_s": packets / t_grad, "heat_group_s": t_group, "heat_serial_s": t_serial,
+            "heat_group_voxel_steps_per_s": voxels * heat_steps / t_group, "adjoint_s": t_adj, "damage_sum": J}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -274,10 +348,10 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-heat", action="store_true")
     ap.add_argument("--workload", default="layered_tissue_200_fresnel_tophat",
-                    choices=["layered_tissue_200_fresnel_tophat", "spectral64", "heat200", "grad128", "adjoint128"],
+                    choices=["layered_tissue_200_fresnel_tophat", "spectral64", "heat200", "grad128", "adjoint128", "sweep32"],
                     help="secondary workloads (each prints its own line): spectral64 batched vs plain launches of the "
                          "64-wavelength workload; heat200 the heat propagator alone; grad128 absorbed-power gradients; "
-                         "adjoint128 the heat adjoint")
+                         "adjoint128 the heat adjoint; sweep32 the 32-grid forward + backward sweep")
     ap.add_argument("--copy-probe", type=int, default=0, metavar="BYTES",
                     help="only run device-to-device copies of BYTES (counter calibration) and exit")
     args = ap.parse_args()
@@ -285,7 +359,8 @@ def main():
         print("bench.py measures one GPU only: this build has no multi-GPU path (run with --gpus 1)", file=sys.stderr)
         sys.exit(2)
 
-    secondary = {"spectral64": bench_spectral, "heat200": bench_heat_line, "grad128": bench_grad, "adjoint128": bench_adjoint}
+    secondary = {"spectral64": bench_spectral, "heat200": bench_heat_line, "grad128": bench_grad, "adjoint128": bench_adjoint,
+                 "sweep32": bench_sweep}
     if args.workload in secondary:
         print(json.dumps(secondary[args.workload]()), flush=True)
         return

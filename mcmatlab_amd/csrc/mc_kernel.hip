@@ -91,10 +91,24 @@ struct McBatch {
   const McParams* params;         // [nModels] (launch and escape tallies)
   const McGeom* geom;             // [nModels]
   const int64_t* off;             // [nItems * 9] offsets of the item's NFR, image, FF, NI[0..5] slices
+  unsigned long long* next;       // [nItems] next global packet index of each item in this launch
+  const unsigned long long* end;  // [nItems] end of each item's packets in this launch
 };
 
-// The parameters a lane uses: the launch's, or in a multi-grid launch the
-// lane's own copy, which each fetch points at the packet's model.
+// Per-launch packet ranges of a multi-grid launch: item i's packets in
+// [first, first + n) are [next[i], end[i]).
+__global__ void mc_multi_ranges(McBatch bt, unsigned long long first, unsigned long long n, unsigned long long* next,
+                                unsigned long long* end) {
+  for (int i = threadIdx.x; i < bt.nItems; i += blockDim.x) {
+    const unsigned long long lo = bt.cum[i] > first ? bt.cum[i] : first;
+    const unsigned long long hi = bt.cum[i + 1] < first + n ? bt.cum[i + 1] : first + n;
+    next[i] = lo;
+    end[i] = hi > lo ? hi : lo;
+  }
+}
+
+// The parameters a wavefront uses: the launch's, or in a multi-grid launch
+// the wavefront's own copy, pointed at the model of its current item.
 template <int KIND>
 struct LaneParams {
   __device__ static const McParams& get(const McParams& p, McParams&) { return p; }
@@ -103,6 +117,36 @@ template <>
 struct LaneParams<kMulti> {
   __device__ static const McParams& get(const McParams&, McParams& q) { return q; }
 };
+
+// Point a multi-grid wavefront at item `it` (wavefront-uniform, so all of
+// this lives in scalar registers).
+template <class Engine, class Sink>
+__device__ inline __attribute__((always_inline)) void multi_item(const McBatch& bt, int it, McParams& q, const Medium*& med,
+                                                                 const uint8_t*& cdf, const Medium* sMed, const uint8_t* sCdf,
+                                                                 Engine& E, Sink& sink) {
+  const McGeom& G = bt.geom[bt.model[it]];
+  for (int a = 0; a < 3; ++a) {
+    q.n[a] = G.n[a];
+    q.d[a] = G.d[a];
+    q.inv_d[a] = G.inv_d[a];
+  }
+  q.tooOld = G.tooOld;
+  q.boundaryType = G.boundaryType;
+  q.calcNFR = G.calcNFR;
+  q.needTime = G.needTime;
+  q.useLC = G.useLC;
+  q.farFieldRes = G.farFieldRes;
+  q.M = G.M;
+  q.normalRI = G.normalRI;
+  q.cdfs = G.cdfs;
+  med = sMed + it * bt.nM;
+  cdf = sCdf + it * bt.nM;
+  E.R.key0 = bt.key[2 * it];
+  E.R.key1 = bt.key[2 * it + 1];
+  E.R.stream = bt.stream[it];
+  sink.NFR = bt.NFR + bt.off[9 * (size_t)it];
+  sink.item = it;
+}
 
 // item owning global packet index g: the last i with cum[i] <= g
 __device__ inline int batch_item(unsigned long long g, const unsigned long long* cum, int n) {
@@ -289,7 +333,7 @@ __global__ void __launch_bounds__(256) mc_tally_escapes(McParams p, McLaunch lp,
 }
 
 template <bool FRESNEL, bool CRITERIA, int BATCH>
-__global__ void __launch_bounds__(256) mc_transport(McParams p, McLaunch lp, McBatch bt, const LaunchRecord* __restrict__ launches,
+__global__ void __launch_bounds__(256, BATCH == kMulti ? 4 : 1) mc_transport(McParams p, McLaunch lp, McBatch bt, const LaunchRecord* __restrict__ launches,
                                                     EscapeRecord* __restrict__ escapes) {
   static_assert(!(BATCH != kPlain && CRITERIA), "batched launches run without deposition criteria");
   constexpr int kMed = BATCH != kPlain ? kBatchMaxMedia : 256;
@@ -320,8 +364,22 @@ __global__ void __launch_bounds__(256) mc_transport(McParams p, McLaunch lp, McB
   const uint8_t* cdf = sCdf;
   unsigned long long nLaunched = 0, nSub = 0, nScat = 0, nIf = 0;
   McParams q;
-  if constexpr (BATCH == kMulti) q = p;
   const McParams& pr = LaneParams<BATCH>::get(p, q);
+  // Multi-grid launches: every lane of a wavefront holds packets of one
+  // item (the wavefront's current item), so the item's grid and tables are
+  // wavefront-uniform.  Wavefronts start spread over the launch's packets
+  // (so items share the GPU in proportion to their packets and their tails
+  // overlap); once their item has no packets left they drain and move on to
+  // the next item that has.
+  int cur = 0;
+  bool curDone = false;
+  if constexpr (BATCH == kMulti) {
+    q = p;
+    const unsigned long long nW = (unsigned long long)gridDim.x * (blockDim.x / 64);
+    const unsigned long long w = (unsigned long long)blockIdx.x * (blockDim.x / 64) + (threadIdx.x >> 6);
+    cur = __builtin_amdgcn_readfirstlane(batch_item(lp.firstPacket + lp.nPackets * w / nW, sCum, bt.nItems));
+    multi_item(bt, cur, q, med, cdf, sMed, sCdf, E, sink);
+  }
 
   for (;;) {
     const int st = E.state;
@@ -329,9 +387,29 @@ __global__ void __launch_bounds__(256) mc_transport(McParams p, McLaunch lp, McB
     const unsigned long long bMove = __ballot(st == ST_MOVE);
     const unsigned long long bScat = __ballot(st == ST_SCATTER);
     const unsigned long long bFres = FRESNEL ? __ballot(st == ST_FRESNEL) : 0ull;
-    const int cF = __popcll(bFetch), cM = __popcll(bMove), cS = __popcll(bScat), cR = __popcll(bFres);
+    const int cM = __popcll(bMove), cS = __popcll(bScat), cR = __popcll(bFres);
+    // lanes waiting on an exhausted item are idle, not fetching
+    const int cF = (BATCH == kMulti && curDone) ? 0 : __popcll(bFetch);
     const int active = cF + cM + cS + cR;
-    if (active == 0) break;
+    if (active == 0) {
+      if constexpr (BATCH == kMulti) {
+        if (curDone && !__hip_atomic_load(&lp.ctr->aborted, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)) {
+          int nxt = -1;
+          for (int j = 1; j <= bt.nItems && nxt < 0; ++j) {
+            const int c = cur + j < bt.nItems ? cur + j : cur + j - bt.nItems;
+            const unsigned long long v = __hip_atomic_load(&bt.next[c], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            if (__builtin_amdgcn_readfirstlane((unsigned)(v < bt.end[c]))) nxt = c;
+          }
+          if (nxt >= 0) {
+            cur = nxt;
+            curDone = false;
+            multi_item(bt, cur, q, med, cdf, sMed, sCdf, E, sink);
+            continue;
+          }
+        }
+      }
+      break;
+    }
     // one state per iteration, chosen by the wavefront (see file comment)
     int phase = ST_MOVE;
     if ((float)cM < lp.moveFrac * (float)active) {
@@ -359,6 +437,31 @@ __global__ void __launch_bounds__(256) mc_transport(McParams p, McLaunch lp, McB
       // consecutive packet numbers for the waiting lanes, one atomic per wavefront
       const int leader = __ffsll((long long)bFetch) - 1;
       unsigned long long base = 0;
+      if constexpr (BATCH == kMulti) {
+        const unsigned long long end = bt.end[cur];
+        if (lane == leader) {
+          base = atomicAdd(&bt.next[cur], (unsigned long long)cF);
+          if (atomicAdd(&lp.ctr->aborted, 0u)) base = end;
+        }
+        base = bcast64(base, leader);
+        curDone = base + (unsigned long long)cF >= end;
+        if (st == ST_FETCH) {
+          const unsigned long long g = base + (unsigned long long)__popcll(bFetch & ((1ull << lane) - 1ull));
+          if (g < end) {
+            const unsigned long long k = g - lp.firstPacket;
+            E.replay = false;
+            E.slot = (uint32_t)k;
+            const LaunchRecord L = launches[k];
+            if (E.start_from(g - sCum[cur], L, pr, med, cdf)) {
+              ++nLaunched;
+            } else {
+              atomicOr(&lp.ctr->aborted, 1u);
+              E.state = ST_DONE;
+            }
+          }
+        }
+        continue;
+      }
       if (lane == leader) {
         base = atomicAdd(&lp.ctr->nextPacket, (unsigned long long)cF);
         if (atomicAdd(&lp.ctr->aborted, 0u)) base = lp.nPackets;  // a failed launch stops the run
@@ -382,26 +485,7 @@ __global__ void __launch_bounds__(256) mc_transport(McParams p, McLaunch lp, McB
             E.R.key0 = bt.key[2 * it];
             E.R.key1 = bt.key[2 * it + 1];
             E.R.stream = bt.stream[it];
-            if constexpr (BATCH == kMulti) {
-              const McGeom& G = bt.geom[bt.model[it]];
-              for (int a = 0; a < 3; ++a) {
-                q.n[a] = G.n[a];
-                q.d[a] = G.d[a];
-                q.inv_d[a] = G.inv_d[a];
-              }
-              q.tooOld = G.tooOld;
-              q.boundaryType = G.boundaryType;
-              q.calcNFR = G.calcNFR;
-              q.needTime = G.needTime;
-              q.useLC = G.useLC;
-              q.farFieldRes = G.farFieldRes;
-              q.M = G.M;
-              q.normalRI = G.normalRI;
-              q.cdfs = G.cdfs;
-              sink.NFR = bt.NFR + bt.off[9 * (size_t)it];
-            } else {
-              sink.NFR = bt.NFR + (size_t)it * bt.L;
-            }
+            sink.NFR = bt.NFR + (size_t)it * bt.L;
             sink.item = it;
             const LaunchRecord L = launches[k];
             ok = E.start_from(g - sCum[it], L, pr, med, cdf);
@@ -667,6 +751,8 @@ int launch_transport(mcm_mc_plan* pl, uint64_t first, uint64_t n, uint64_t seed,
   const bool tally = kind == kMulti ? tallies : (T.base.useLC || T.base.farFieldRes || anyNI);
   if (kind == kMulti) {
     const dim3 g(pl->blocksMulti);
+    hipLaunchKernelGGL(mc_multi_ranges, dim3(1), dim3(256), 0, s, B, (unsigned long long)first, (unsigned long long)n, B.next,
+                       const_cast<unsigned long long*>(B.end));
     hipLaunchKernelGGL(mc_launch_packets<kMulti>, g, b, 0, s, pl->prm, lp, B, L, E);
     if (fr) hipLaunchKernelGGL((mc_transport<true, false, kMulti>), g, b, 0, s, pl->prm, lp, B, L, E);
     else hipLaunchKernelGGL((mc_transport<false, false, kMulti>), g, b, 0, s, pl->prm, lp, B, L, E);
@@ -1184,11 +1270,12 @@ int mcm_mc_multi_run(mcm_mc_plan* const* plans, int64_t nPlans, const mcm_mc_mul
         cdf[(size_t)j * nMmax + m] = T.cdfIdx[(size_t)it.iL * 256 + m];
       }
     }
-    DevBuf dCum, dKey, dStrm, dModel, dOffc, dMed, dCdf, dCollc;
+    DevBuf dCum, dKey, dStrm, dModel, dOffc, dMed, dCdf, dCollc, dNext, dEnd;
     if ((rc = upload(dCum, cum.data(), cum.size(), s)) || (rc = upload(dKey, key.data(), key.size(), s)) ||
         (rc = upload(dStrm, strm.data(), strm.size(), s)) || (rc = upload(dModel, model.data(), model.size(), s)) ||
         (rc = upload(dOffc, offc.data(), offc.size(), s)) || (rc = upload(dMed, med.data(), med.size(), s)) ||
-        (rc = upload(dCdf, cdf.data(), cdf.size(), s)) || (rc = zeros(dCollc, (size_t)n * sizeof(unsigned long long), s)))
+        (rc = upload(dCdf, cdf.data(), cdf.size(), s)) || (rc = zeros(dCollc, (size_t)n * sizeof(unsigned long long), s)) ||
+        (rc = zeros(dNext, (size_t)n * sizeof(unsigned long long), s)) || (rc = zeros(dEnd, (size_t)n * sizeof(unsigned long long), s)))
       return rc;
     McBatch B;
     std::memset(&B, 0, sizeof B);
@@ -1208,11 +1295,20 @@ int mcm_mc_multi_run(mcm_mc_plan* const* plans, int64_t nPlans, const mcm_mc_mul
     B.params = dParams.as<McParams>();
     B.geom = dGeom.as<McGeom>();
     B.off = dOffc.as<int64_t>();
+    B.next = dNext.as<unsigned long long>();
+    B.end = dEnd.as<unsigned long long>();
+    // launches of whole items where they fit (an item split over two
+    // launches pays the tail of its longest-lived packets twice)
     const unsigned long long total = cum[(size_t)n];
     for (unsigned long long done = 0; done < total;) {
-      const uint64_t m = std::min<unsigned long long>(total - done, kMaxLaunchPackets);
-      if ((rc = launch_transport(pl0, done, m, 0, 0, s, &B, kMulti, cls == 0, tallies))) return rc;
-      done += m;
+      unsigned long long stop = done + std::min<unsigned long long>(total - done, kMaxLaunchPackets);
+      for (int j = n; j > 0; --j)
+        if (cum[(size_t)j] <= stop && cum[(size_t)j] > done) {
+          stop = cum[(size_t)j] - done >= kMaxLaunchPackets / 4 || cum[(size_t)j] == total ? cum[(size_t)j] : stop;
+          break;
+        }
+      if ((rc = launch_transport(pl0, done, stop - done, 0, 0, s, &B, kMulti, cls == 0, tallies))) return rc;
+      done = stop;
     }
     std::vector<unsigned long long> cc((size_t)n);
     MCM_HIP(hipMemcpyAsync(cc.data(), dCollc.p, cc.size() * sizeof(unsigned long long), hipMemcpyDeviceToHost, s));

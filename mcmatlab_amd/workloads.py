@@ -169,3 +169,74 @@ def layered_tissue_heat(n=200, side=SIDE, power=1.0, NFR=None, steps_on=667, ste
     T0 = np.full((n, n, n), 37.0, np.float32)
     Om0 = np.zeros((n, n, n), np.float32)
     return T0, Om0, P_on, P_off
+
+
+def multi_grid_sides(n_models=32, seed=3, lo=64, hi=256):
+    """Grid sides log-uniform in [lo, hi] from numpy.random.default_rng(seed)."""
+    rng = np.random.default_rng(seed)
+    return np.round(np.exp(rng.uniform(math.log(lo), math.log(hi), n_models))).astype(int)
+
+
+def multi_grid_sweep(n_models=32, seed=3, side=SIDE):
+    """Independent heterogeneous models on uneven grids (sides from
+    multi_grid_sides(n_models, seed)): a scattering background (mua 1/cm,
+    mus 100/cm, g 0.9, n 1.37) with 1-6 ellipsoid inclusions (each its own
+    medium: mua U[0.5, 100] /cm, mus U[50, 500] /cm, g U[0.7, 0.95],
+    n U[1.33, 1.45]; centres and semi-axes uniform in the cube), cyclic x/y
+    boundaries, a pencil beam along z at the centre and 1e6-1e7 packets
+    (log-uniform).  Returns a list of dicts: model, nPhotons, box (the
+    gradient's target box, 0-based half-open voxel ranges), heat (a function
+    NFR -> (T0, Omega0, P) for 1000 steps of 5 us with the light on,
+    inclusions carrying the Arrhenius parameters of blood)."""
+    rng = np.random.default_rng(seed + 1000)
+    out = []
+    for n in multi_grid_sides(n_models, seed):
+        n = int(n)
+        d = side / n
+        c = (np.arange(n) + 0.5) * d
+        X, Y, Z = np.meshgrid(c, c, c, indexing="ij")
+        M = np.ones((n, n, n), np.uint8)
+        nInc = int(rng.integers(1, 7))
+        mua, mus, g, ri = [1.0], [100.0], [0.9], [1.37]
+        for k in range(nInc):
+            ctr = rng.uniform(0.15, 0.85, 3) * side
+            ax = rng.uniform(0.05, 0.25, 3) * side
+            inside = ((X - ctr[0]) / ax[0]) ** 2 + ((Y - ctr[1]) / ax[1]) ** 2 + ((Z - ctr[2]) / ax[2]) ** 2 <= 1
+            M[inside] = k + 2
+            mua.append(rng.uniform(0.5, 100))
+            mus.append(rng.uniform(50, 500))
+            g.append(rng.uniform(0.7, 0.95))
+            ri.append(rng.uniform(1.33, 1.45))
+        m = Model()
+        m.G = Geometry(dx=d, dy=d, dz=d)
+        mc = m.MC
+        mc.M = M
+        mc.mediaProperties = media_table(mua, mus, g, ri)
+        mc.interfaceNormals = interface_normals(M, ri, (d, d, d))
+        mc.boundaryType = 3
+        mc.spectrum = np.array([1.0])
+        mc.LS.sourceType = 0
+        mc.LS.xFocus = mc.LS.yFocus = 0.0
+        mc.LS.zFocus = 0.0
+        nph = int(round(math.exp(rng.uniform(math.log(1e6), math.log(1e7)))))
+        mc.nPhotonsRequested = float(nph)
+        lo, hi = n // 2 - max(n // 16, 1), n // 2 + max(n // 16, 1)
+        box = ((lo, hi), (lo, hi), (n // 8, n // 4))
+        out.append({"model": m, "nPhotons": nph, "box": box, "side": n,
+                    "heat": (lambda NFR, M=M, mua=np.asarray(mua), nM=len(mua), d=d: _sweep_heat(NFR, M, mua, nM, d))})
+    return out
+
+
+def _sweep_heat(NFR, M, mua, nM, d, power=1.0, steps=1000, dt=5e-6):
+    VHC = np.full(nM, 3.7, np.float32)
+    TC = np.full(nM, 0.0037, np.float32)
+    NFR = np.asarray(NFR, np.float32).reshape(M.shape, order="F") if np.ndim(NFR) != 3 else np.asarray(NFR, np.float32)
+    dTdt_abs = (NFR * mua.astype(np.float32)[M - 1] * np.float32(power) / VHC[M - 1]).astype(np.float32)
+    A = np.zeros(nM)
+    E = np.zeros(nM)
+    A[1:], E[1:] = 7.6e66, 422.5e3
+    P = {"M": (M - 1).astype(np.uint8), "A": A, "E": E, "dTdtperdeltaT": heat_coefficients(VHC, TC, d, d, d),
+         "dTdt_abs": dTdt_abs, "useAllCPUs": True, "heatBoundaryType": 1, "tempSensorCornerIdxs": np.zeros(0),
+         "tempSensorInterpWeights": np.zeros((0, 3)), "lightsOn": True, "steps": steps, "dt": dt}
+    T0 = np.full(M.shape, 37.0, np.float32)
+    return T0, np.zeros(M.shape, np.float32), P
