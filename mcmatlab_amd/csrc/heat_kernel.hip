@@ -550,7 +550,10 @@ __global__ void __launch_bounds__(kHeatTPB) heat_sweep_x_wave(HeatArgs<double> a
 // line, in the expression order of the sequential kernel.
 template <int S>
 struct YzTile {
-  static constexpr int TX = S <= 4 ? 16 : 8;  // lines per workgroup
+  // lines per workgroup: 16 (128-byte rows) up to 128 voxels a line; longer
+  // lines would need ~70 KB of LDS per workgroup at 16, leaving only two
+  // workgroups per CU, so they take 8
+  static constexpr int TX = S <= 2 ? 16 : 8;
   static constexpr int NP = 64 * S;           // padded line length
   static constexpr int LD = NP + 1;           // LDS pitch of a line (doubles)
 };
@@ -1429,6 +1432,9 @@ int mcm_heat_plan_create(const mcm_heat_input* in, mcm_heat_plan** out) {
   P->nS = in->nSensors;
   P->interrupt = in->isInterruptPending;
   MCM_HIP(hipStreamCreate(&P->stream));
+  // default steps per captured graph (0: launch every step, e.g. under a
+  // kernel tracer that does not follow graph launches)
+  if (const char* e = std::getenv("MCM_HEAT_GRAPH_STEPS")) P->graphSteps = std::max<int64_t>(0, std::atoll(e));
   const size_t rs = rt_size(P.get());
   if ((rc = halloc(P->M, (size_t)L)) || (rc = halloc(P->Q, (size_t)L * 4)) || (rc = halloc(P->c, (size_t)P->nM * P->nM * 3 * rs)) ||
       (rc = halloc(P->logA, (size_t)P->nM * rs)) || (rc = halloc(P->E, (size_t)P->nM * rs)) || (rc = halloc(P->T1, (size_t)L * rs)) ||
