@@ -265,14 +265,16 @@ def bench_adjoint(n=128, steps=3000, every=100):
             "dJ_dQ_max": float(abs(r["dJ_dQ"]).max())}
 
 
-def bench_sweep(n_models=32, seed=3, heat_steps=1000, every=100):
+def bench_sweep(n_models=32, seed=3, heat_steps=1000, every=100, parts=("mc", "grad", "heat", "adjoint")):
     """Batched forward + backward sweep over independent models on uneven
     grids (multi_grid_sweep: sides 64..247 from seed 3, cyclic x/y, 1e6-1e7
     packets each): forward MC of every model as one multi-grid launch
     sequence (and, for comparison, as plain launches one model after
     another), the absorbed-power gradient of each model, 1000 heat steps of
     every model as one plan group (and one plan after another), and each
-    model's heat adjoint.  Stages print progress to stderr."""
+    model's heat adjoint.  `parts` selects stages (the heat and adjoint
+    stages need "mc" for their heating rates).  Stages print progress to
+    stderr."""
     from mcmatlab_amd import api
     from mcmatlab_amd.workloads import multi_grid_sweep
 
@@ -299,44 +301,56 @@ def bench_sweep(n_models=32, seed=3, heat_steps=1000, every=100):
         p.normalize(iL)
     t_plain = time.perf_counter() - t
     log(f"plain MC {t_plain:.2f}s")
-    t = time.perf_counter()
-    for p, w in zip(plans, W):
-        p.absorbed_power(w["box"], w["nPhotons"], seed=5)
-    t_grad = time.perf_counter() - t
-    log(f"gradients {t_grad:.2f}s")
+    res = {"metric": f"seconds, {n_models}-grid sweep (sides 64..256): " + " + ".join(parts), "unit": "s",
+           "higher_is_better": False, "models": n_models, "voxels": voxels, "packets": packets, "mc_multi_s": t_multi,
+           "mc_plain_s": t_plain, "mc_multi_photons_per_s": packets / t_multi}
+    total = t_multi
+    if "grad" in parts:
+        t = time.perf_counter()
+        for p, w in zip(plans, W):
+            p.absorbed_power(w["box"], w["nPhotons"], seed=5)
+        t_grad = time.perf_counter() - t
+        log(f"gradients {t_grad:.2f}s")
+        res.update(grad_s=t_grad, grad_packets_per_s=packets / t_grad)
+        total += t_grad
     for p in plans:
         p.close()
+    if "heat" not in parts and "adjoint" not in parts:
+        res["value"] = total
+        return res
     cases = [w["heat"](o["NFR"][..., 0]) for w, o in zip(W, outs)]
-    hplans = [api.HeatPlan(T0, Om0, P) for T0, Om0, P in cases]
-    dts = [P["dt"] for _, _, P in cases]
-    hip.sync()
-    t = time.perf_counter()
-    with api.HeatGroup(hplans, steps_per_graph=32) as g:
-        g.advance(heat_steps, dts, [1] * len(hplans))
+    if "heat" in parts:
+        hplans = [api.HeatPlan(T0, Om0, P) for T0, Om0, P in cases]
+        dts = [P["dt"] for _, _, P in cases]
         hip.sync()
-    t_group = time.perf_counter() - t
-    log(f"heat group {t_group:.2f}s")
-    t = time.perf_counter()
-    for h, d in zip(hplans, dts):
-        h.advance(heat_steps, d, 1)
-        hip.sync()
-    t_serial = time.perf_counter() - t
-    log(f"heat one plan after another {t_serial:.2f}s")
-    for h in hplans:
-        h.close()
-    t = time.perf_counter()
-    J = 0.0
-    for T0, Om0, P in cases:
-        r = api.heat_adjoint(T0, Om0, P, segments=[(heat_steps, P["dt"], True)], checkpointEvery=every)
-        J += r["J"]
-    t_adj = time.perf_counter() - t
-    log(f"adjoints {t_adj:.2f}s")
-    total = t_multi + t_grad + t_group + t_adj
-    return {"metric": f"seconds, {n_models}-grid sweep (sides 64..256): forward MC + mua/mus gradients + {heat_steps}-step heat + adjoint",
-            "value": total, "unit": "s", "higher_is_better": False, "models": n_models, "voxels": voxels, "packets": packets,
-            "mc_multi_s": t_multi, "mc_plain_s": t_plain, "mc_multi_photons_per_s": packets / t_multi, "grad_s": t_grad,
-            "grad_packets_per_s": packets / t_grad, "heat_group_s": t_group, "heat_serial_s": t_serial,
-            "heat_group_voxel_steps_per_s": voxels * heat_steps / t_group, "adjoint_s": t_adj, "damage_sum": J}
+        t = time.perf_counter()
+        with api.HeatGroup(hplans, steps_per_graph=32) as g:
+            g.advance(heat_steps, dts, [1] * len(hplans))
+            hip.sync()
+        t_group = time.perf_counter() - t
+        log(f"heat group {t_group:.2f}s")
+        t = time.perf_counter()
+        for h, d in zip(hplans, dts):
+            h.advance(heat_steps, d, 1)
+            hip.sync()
+        t_serial = time.perf_counter() - t
+        log(f"heat one plan after another {t_serial:.2f}s")
+        for h in hplans:
+            h.close()
+        res.update(heat_group_s=t_group, heat_serial_s=t_serial, heat_group_voxel_steps_per_s=voxels * heat_steps / t_group)
+        total += t_group
+    if "adjoint" in parts:
+        t = time.perf_counter()
+        J = 0.0
+        for T0, Om0, P in cases:
+            r = api.heat_adjoint(T0, Om0, P, segments=[(heat_steps, P["dt"], True)], checkpointEvery=every)
+            J += r["J"]
+        t_adj = time.perf_counter() - t
+        log(f"adjoints {t_adj:.2f}s")
+        res.update(adjoint_s=t_adj, damage_sum=J)
+        total += t_adj
+    res["value"] = total
+    return res
 
 
 def main():
@@ -352,6 +366,8 @@ def main():
                     help="secondary workloads (each prints its own line): spectral64 batched vs plain launches of the "
                          "64-wavelength workload; heat200 the heat propagator alone; grad128 absorbed-power gradients; "
                          "adjoint128 the heat adjoint; sweep32 the 32-grid forward + backward sweep")
+    ap.add_argument("--sweep-parts", default="mc,grad,heat,adjoint",
+                    help="stages of the sweep32 workload (comma-separated subset of mc,grad,heat,adjoint)")
     ap.add_argument("--copy-probe", type=int, default=0, metavar="BYTES",
                     help="only run device-to-device copies of BYTES (counter calibration) and exit")
     args = ap.parse_args()
@@ -362,7 +378,8 @@ def main():
     secondary = {"spectral64": bench_spectral, "heat200": bench_heat_line, "grad128": bench_grad, "adjoint128": bench_adjoint,
                  "sweep32": bench_sweep}
     if args.workload in secondary:
-        print(json.dumps(secondary[args.workload]()), flush=True)
+        kw = {"parts": tuple(args.sweep_parts.split(","))} if args.workload == "sweep32" else {}
+        print(json.dumps(secondary[args.workload](**kw)), flush=True)
         return
     if args.copy_probe:
         Hip().copy_probe(args.copy_probe)
