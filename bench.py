@@ -366,6 +366,8 @@ def main():
                     help="secondary workloads (each prints its own line): spectral64 batched vs plain launches of the "
                          "64-wavelength workload; heat200 the heat propagator alone; grad128 absorbed-power gradients; "
                          "adjoint128 the heat adjoint; sweep32 the 32-grid forward + backward sweep")
+    ap.add_argument("--copies", type=int, default=0, choices=[0, 1, 2, 4, 8],
+                    help="force the number of XCD-private fluence copies (0: the autotuner's choice)")
     ap.add_argument("--sweep-parts", default="mc,grad,heat,adjoint",
                     help="stages of the sweep32 workload (comma-separated subset of mc,grad,heat,adjoint)")
     ap.add_argument("--copy-probe", type=int, default=0, metavar="BYTES",
@@ -397,6 +399,9 @@ def main():
     tune_ms = plan.autotune()  # number of XCD-private fluence copies for this model
     best = min(t for t in tune_ms.values() if t > 0)
     copies = min(R for R, t in tune_ms.items() if 0 < t <= 1.10 * best)
+    if args.copies:
+        copies = args.copies
+        plan.set_fluence_copies(copies)
     print(f"[bench] setup {time.perf_counter() - t_setup:.1f}s, fluence copies {copies} (trial ms {tune_ms})", file=sys.stderr,
           flush=True)
     npk = int(args.packets)

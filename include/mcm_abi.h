@@ -209,6 +209,10 @@ int mcm_mc_plan_destroy(mcm_mc_plan *plan);
  * ms[4] (optional) receives the trial times for 1, 2, 4, 8 copies (0 =
  * skipped).  mcm_mc_run calls it for runs of 2e7 packets or more. */
 int mcm_mc_plan_autotune(mcm_mc_plan *plan, int64_t iL, uint64_t nPackets, double ms[4]);
+/* Set the number of XCD-private FP64 fluence copies (1, 2, 4 or 8) by hand;
+ * tallies so far are folded into the first copy and kept.  No reference
+ * counterpart. */
+int mcm_mc_plan_set_fluence_copies(mcm_mc_plan *plan, int32_t copies);
 
 /* Batched launch (no reference counterpart: the reference runs one
  * wavelength after another, MCmatlab.c:619-747).  Item i asks for nPhotons

@@ -24,6 +24,7 @@ SIGNATURES = {
     "mcm_mc_plan_counters": (C.c_int, [C.c_void_p, C.POINTER(C.c_double)]),
     "mcm_mc_plan_destroy": (C.c_int, [C.c_void_p]),
     "mcm_mc_plan_autotune": (C.c_int, [C.c_void_p, C.c_int64, C.c_uint64, C.POINTER(C.c_double)]),
+    "mcm_mc_plan_set_fluence_copies": (C.c_int, [C.c_void_p, C.c_int32]),
     "mcm_mc_batch_run": (C.c_int, [C.c_void_p, C.POINTER(abi.mcm_mc_item), C.c_int64, C.POINTER(abi.mcm_mc_output)]),
     "mcm_mc_multi_run": (C.c_int, [C.POINTER(C.c_void_p), C.c_int64, C.POINTER(abi.mcm_mc_multi_item), C.c_int64,
                                    C.POINTER(abi.mcm_mc_output)]),

@@ -269,6 +269,10 @@ class MCPlan:
         check(lib().mcm_mc_plan_autotune(self.h, int(iL), int(nPackets), buf))
         return dict(zip((1, 2, 4, 8), buf[:]))
 
+    def set_fluence_copies(self, copies):
+        """mcm_mc_plan_set_fluence_copies: 1, 2, 4 or 8 XCD-private copies."""
+        check(lib().mcm_mc_plan_set_fluence_copies(self.h, int(copies)))
+
     def batch(self, items):
         """mcm_mc_batch_run: items = [(iL, nPhotons, seed, stream), ...] run
         in one sequence of persistent launches; returns one output dict per

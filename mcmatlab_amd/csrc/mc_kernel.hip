@@ -1054,6 +1054,14 @@ int mcm_mc_plan_autotune(mcm_mc_plan* pl, int64_t iL, uint64_t nPackets, double*
   return MCM_OK;
 }
 
+int mcm_mc_plan_set_fluence_copies(mcm_mc_plan* pl, int32_t copies) {
+  if (!pl) return fail(MCM_ERR_INVALID_ARGUMENT, "null plan");
+  if (copies != 1 && copies != 2 && copies != 4 && copies != 8) return fail(MCM_ERR_INVALID_ARGUMENT, "fluence copies must be 1, 2, 4 or 8");
+  if (copies > 1 && !pl->T.calcNFR) return fail(MCM_ERR_INVALID_ARGUMENT, "fluence copies need calcNFR");
+  MCM_HIP(hipSetDevice(pl->device));
+  return set_replicas(pl, copies);
+}
+
 int mcm_mc_plan_counters(mcm_mc_plan* pl, double* out) {
   if (!pl || !out) return fail(MCM_ERR_INVALID_ARGUMENT, "null argument");
   MCM_HIP(hipSetDevice(pl->device));

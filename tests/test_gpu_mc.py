@@ -134,3 +134,28 @@ def test_gpu_fluence_copies_keep_results(api, name):
         b.close()
     for k in ("NFR", "NI_zneg", "NI_zpos"):
         np.testing.assert_allclose(rb[k], ra[k], rtol=1e-5, atol=1e-6 * float(np.max(np.abs(ra[k]))))
+
+
+@pytest.mark.parametrize("copies", [2, 8])
+def test_gpu_set_fluence_copies(api, copies):
+    """Copies set by hand: deposits made before the change are folded and
+    kept, and the outputs equal a single-copy plan's."""
+    m = models.cases()["pencil_bt1"]()
+    a = api.MCPlan(m)
+    b = api.MCPlan(m)
+    try:
+        with pytest.raises(RuntimeError):
+            b.set_fluence_copies(3)
+        for plan in (a, b):
+            plan.simulate(10000, seed=6, firstPhoton=0)
+        b.set_fluence_copies(copies)
+        for plan in (a, b):
+            plan.simulate(20000, seed=6, firstPhoton=10000)
+        b.set_fluence_copies(1)
+        b.set_fluence_copies(copies)
+        ra, rb = a.normalize(), b.normalize()
+    finally:
+        a.close()
+        b.close()
+    for k in ("NFR", "NI_zneg", "NI_zpos"):
+        np.testing.assert_allclose(rb[k], ra[k], rtol=1e-5, atol=1e-6 * float(np.max(np.abs(ra[k]))))
