@@ -903,11 +903,58 @@ __global__ void __launch_bounds__(kHeatTPB) heat_adj_solve(HeatArgs<double> a, c
   }
 }
 
-// A_d^T y = r with one wavefront per line (lines of up to 512 voxels): the
-// rows of A^T are (sub, diag, super) = (c_(k-1), b_k, a_(k+1)) of A's rows.
+// A_d^T x = r along one line of n voxels held S per lane by a wavefront
+// (lane l: voxels l*S .. l*S+S-1; rv the right-hand side, Mv the media; lf:
+// the whole line is frozen).  The rows of A^T are (sub, diag, super) =
+// (c_(k-1), b_k, a_(k+1)) of A's rows; the neighbouring lane's edge
+// coefficients come by one shuffle each.
+template <int S>
+__device__ inline void adj_line_solve(const HeatArgs<double>& a, int axis, int n, bool lf, const double (&rv)[S], const int (&Mv)[S],
+                                      double (&xs)[S]) {
+  const int lane = __lane_id();
+  const double h = a.dt / 2;
+  double A[S], B[S], Cc[S];
+  const int Ml = __shfl(Mv[S - 1], lane > 0 ? lane - 1 : 0, 64), Mr = __shfl(Mv[0], lane < 63 ? lane + 1 : 63, 64);
+  // rows of A
+#pragma unroll
+  for (int s = 0; s < S; ++s) {
+    const int k = lane * S + s;
+    A[s] = 0.0;
+    B[s] = 1.0;
+    Cc[s] = 0.0;
+    if (k >= n || lf || (a.sinked && (k == 0 || k == n - 1))) continue;
+    const int m = Mv[s];
+    const int ML = s > 0 ? Mv[s - 1] : Ml, MR = s < S - 1 ? Mv[s + 1] : Mr;
+    if (k > 0) {
+      const double q = h * cc(a, m, ML, axis);
+      B[s] += q;
+      A[s] = -q;
+    }
+    if (k < n - 1) {
+      const double q = h * cc(a, m, MR, axis);
+      B[s] += q;
+      Cc[s] = -q;
+    }
+  }
+  // rows of A^T
+  const double cPrevLane = lane_shfl(Cc[S - 1], lane - 1), aNextLane = lane_shfl(A[0], lane + 1);
+  double ta[S], tb[S], tc[S];
+#pragma unroll
+  for (int s = 0; s < S; ++s) {
+    const int k = lane * S + s;
+    const bool ok = k < n;
+    ta[s] = ok && k > 0 ? (s > 0 ? Cc[s - 1] : cPrevLane) : 0.0;
+    tb[s] = ok ? B[s] : 1.0;
+    tc[s] = ok && k < n - 1 ? (s < S - 1 ? A[s + 1] : aNextLane) : 0.0;
+  }
+  wave_tridiag<S>(ta, tb, tc, rv, xs);
+}
+
+// A_d^T y = r with one wavefront per line (lines of up to 512 voxels),
+// loading the line directly (x lines; strided along y and z).
 template <int S, int AXIS>
 __global__ void __launch_bounds__(kHeatTPB) heat_adj_solve_wave(HeatArgs<double> a, const double* __restrict__ r, double* y) {
-  const int nx = a.nx, ny = a.ny, nz = a.nz, nxy = nx * ny, nM = a.nM;
+  const int nx = a.nx, ny = a.ny, nz = a.nz, nxy = nx * ny;
   const int lane = __lane_id();
   const int line = blockIdx.x * (kHeatTPB / 64) + (threadIdx.x >> 6);
   int i0, st, n, nLines;
@@ -929,8 +976,7 @@ __global__ void __launch_bounds__(kHeatTPB) heat_adj_solve_wave(HeatArgs<double>
     lf = a.sinked && (xx == 0 || xx == nx - 1 || yy == 0 || yy == ny - 1);
   }
   if (line >= nLines) return;
-  const double h = a.dt / 2;
-  double rv[S], A[S], B[S], Cc[S];
+  double rv[S], xs[S];
   int Mv[S];
 #pragma unroll
   for (int s = 0; s < S; ++s) {
@@ -939,44 +985,79 @@ __global__ void __launch_bounds__(kHeatTPB) heat_adj_solve_wave(HeatArgs<double>
     rv[s] = k < n ? r[i] : 0.0;
     Mv[s] = a.M[i];
   }
-  const int Ml = __shfl(Mv[S - 1], lane > 0 ? lane - 1 : 0, 64), Mr = __shfl(Mv[0], lane < 63 ? lane + 1 : 63, 64);
-  // rows of A
-#pragma unroll
-  for (int s = 0; s < S; ++s) {
-    const int k = lane * S + s;
-    A[s] = 0.0;
-    B[s] = 1.0;
-    Cc[s] = 0.0;
-    if (k >= n || lf || (a.sinked && (k == 0 || k == n - 1))) continue;
-    const int m = Mv[s];
-    const int ML = s > 0 ? Mv[s - 1] : Ml, MR = s < S - 1 ? Mv[s + 1] : Mr;
-    if (k > 0) {
-      const double q = h * cc(a, m, ML, AXIS);
-      B[s] += q;
-      A[s] = -q;
-    }
-    if (k < n - 1) {
-      const double q = h * cc(a, m, MR, AXIS);
-      B[s] += q;
-      Cc[s] = -q;
-    }
-  }
-  // rows of A^T
-  const double cPrevLane = lane_shfl(Cc[S - 1], lane - 1), aNextLane = lane_shfl(A[0], lane + 1);
-  double ta[S], tb[S], tc[S], xs[S];
-#pragma unroll
-  for (int s = 0; s < S; ++s) {
-    const int k = lane * S + s;
-    const bool ok = k < n;
-    ta[s] = ok && k > 0 ? (s > 0 ? Cc[s - 1] : cPrevLane) : 0.0;
-    tb[s] = ok ? B[s] : 1.0;
-    tc[s] = ok && k < n - 1 ? (s < S - 1 ? A[s + 1] : aNextLane) : 0.0;
-  }
-  wave_tridiag<S>(ta, tb, tc, rv, xs);
+  adj_line_solve<S>(a, AXIS, n, lf, rv, Mv, xs);
 #pragma unroll
   for (int s = 0; s < S; ++s) {
     const int k = lane * S + s;
     if (k < n) y[i0 + k * st] = xs[s];
+  }
+}
+
+// A_d^T y = r along y (AXIS 1) or z (AXIS 2) with the lines staged through
+// LDS like the forward y/z sweeps: a workgroup takes TX consecutive x-lines
+// at one z (AXIS 1) or y (AXIS 2), loads them row by row (coalesced), each
+// wavefront solves TX/4 lines, and the rows are written back.
+template <int S, int AXIS>
+__global__ void __launch_bounds__(kHeatTPB) heat_adj_solve_tile(HeatArgs<double> a, const double* __restrict__ r, double* y) {
+  constexpr int TX = YzTile<S>::TX, NP = YzTile<S>::NP, LD = YzTile<S>::LD;
+  __shared__ double sR[TX * LD];
+  __shared__ uint8_t sM[TX * NP];
+  const int nx = a.nx, ny = a.ny, nz = a.nz, nxy = nx * ny;
+  const int n = AXIS == 1 ? ny : nz;
+  const int st = AXIS == 1 ? nx : nxy;
+  const int nOther = AXIS == 1 ? nz : ny;
+  const int ntx = (nx + TX - 1) / TX;
+  const int tx0 = (blockIdx.x % ntx) * TX, o = blockIdx.x / ntx;
+  const int base = AXIS == 1 ? tx0 + o * nxy : tx0 + o * nx;
+  const int cols = min(TX, nx - tx0);
+  constexpr int kIt = (TX * NP + kHeatTPB - 1) / kHeatTPB;
+  {
+    double vR[kIt];
+    int vM[kIt];
+#pragma unroll
+    for (int q = 0; q < kIt; ++q) {
+      const int e = threadIdx.x + q * kHeatTPB;
+      const int c = e % TX, k = e / TX;
+      const int i = c < cols && k < n ? base + c + k * st : base;
+      vR[q] = r[i];
+      vM[q] = a.M[i];
+    }
+#pragma unroll
+    for (int q = 0; q < kIt; ++q) {
+      const int e = threadIdx.x + q * kHeatTPB;
+      const int c = e % TX, k = e / TX;
+      if (c < cols && k < n) {
+        sR[c * LD + k] = vR[q];
+        sM[c * NP + k] = (uint8_t)vM[q];
+      }
+    }
+  }
+  __syncthreads();
+  const int lane = __lane_id(), wave = threadIdx.x >> 6;
+  for (int c = wave; c < cols; c += kHeatTPB / 64) {
+    const int x = tx0 + c;
+    const bool lf = a.sinked && (x == 0 || x == nx - 1 || o == 0 || o == nOther - 1);
+    double rv[S], xs[S];
+    int Mv[S];
+#pragma unroll
+    for (int s = 0; s < S; ++s) {
+      const int k = lane * S + s;
+      rv[s] = k < n ? sR[c * LD + k] : 0.0;
+      Mv[s] = k < n ? (int)sM[c * NP + k] : 0;
+    }
+    adj_line_solve<S>(a, AXIS, n, lf, rv, Mv, xs);
+#pragma unroll
+    for (int s = 0; s < S; ++s) {
+      const int k = lane * S + s;
+      if (k < n) sR[c * LD + k] = xs[s];
+    }
+  }
+  __syncthreads();
+#pragma unroll
+  for (int q = 0; q < kIt; ++q) {
+    const int e = threadIdx.x + q * kHeatTPB;
+    const int c = e % TX, k = e / TX;
+    if (c < cols && k < n) y[base + c + k * st] = sR[c * LD + k];
   }
 }
 
@@ -1379,6 +1460,21 @@ void adj_solve(mcm_heat_plan* P, const HeatArgs<double>& a, const double* r, dou
   const int n = AXIS == 0 ? P->nx : AXIS == 1 ? P->ny : P->nz;
   const int64_t lines = AXIS == 0 ? (int64_t)P->ny * P->nz : AXIS == 1 ? (int64_t)P->nx * P->nz : (int64_t)P->nx * P->ny;
   const unsigned gw = (unsigned)((lines + kHeatTPB / 64 - 1) / (kHeatTPB / 64));
+  if (AXIS != 0 && wave_rows(n)) {
+    // y and z lines through LDS tiles
+    auto tile = [&](auto sTag) {
+      constexpr int S = decltype(sTag)::value;
+      constexpr int TX = YzTile<S>::TX;
+      const unsigned g = (unsigned)(((P->nx + TX - 1) / TX) * (AXIS == 1 ? P->nz : P->ny));
+      hipLaunchKernelGGL((heat_adj_solve_tile<S, AXIS == 1 ? 1 : 2>), dim3(g), dim3(kHeatTPB), 0, P->stream, a, r, y);
+    };
+    switch (wave_rows(n)) {
+      case 1: tile(std::integral_constant<int, 1>()); return;
+      case 2: tile(std::integral_constant<int, 2>()); return;
+      case 4: tile(std::integral_constant<int, 4>()); return;
+      default: tile(std::integral_constant<int, 8>()); return;
+    }
+  }
   switch (wave_rows(n)) {
     case 1: hipLaunchKernelGGL((heat_adj_solve_wave<1, AXIS>), dim3(gw), dim3(kHeatTPB), 0, P->stream, a, r, y); return;
     case 2: hipLaunchKernelGGL((heat_adj_solve_wave<2, AXIS>), dim3(gw), dim3(kHeatTPB), 0, P->stream, a, r, y); return;
