@@ -118,31 +118,34 @@ def roofline_counts():
         return json.load(f)["workloads"][WORKLOAD]
 
 
-def cpu_baseline(model, seconds=15.0):
+def cpu_baseline(model, seconds=15.0, seconds_1thread=5.0):
     """FP64 oracle (OpenMP, dSFMT streams seeded seed+thread) on a bounded
-    sample of the same workload; photons/s on min(16, host cores) threads."""
+    sample of the same workload; photons/s on min(16, host cores) threads,
+    and on one thread for a smaller sample."""
     from mcmatlab_amd import abi
 
     lib = C.CDLL(os.path.join(ROOT, "oracle", "_build", "liboracle_omp.so"))
     lib.oracle_mc_run.argtypes = [C.POINTER(abi.mcm_mc_input), C.POINTER(abi.mcm_mc_output), C.c_int, C.c_int, C.c_uint64, C.c_void_p]
     cores = max(1, min(16, os.cpu_count() or 1))
 
-    def run(n):
+    def run(n, threads):
         model.MC.nPhotonsRequested = float(n)
         inp = abi.MCInputs(model, seed=0)
         out = abi.MCOutputs(inp)
         t = time.perf_counter()
-        rc = lib.oracle_mc_run(C.byref(inp.struct), C.byref(out.struct), cores, 0, 0, None)
+        rc = lib.oracle_mc_run(C.byref(inp.struct), C.byref(out.struct), threads, 0, 0, None)
         dt = time.perf_counter() - t
         if rc != 0:
             raise RuntimeError("oracle failed")
         return out.struct.nPhotons, dt
 
-    n0, dt0 = run(1000 * cores)
+    n0, dt0 = run(1000 * cores, cores)
     n = max(2000, int(n0 / max(dt0, 1e-3) * seconds))
-    n1, dt1 = run(n)
+    n1, dt1 = run(n, cores)
+    n2, dt2 = run(max(500, int(n1 / dt1 / cores * seconds_1thread)), 1)
     return {"value": n1 / dt1, "unit": "photons/s", "cores": cores, "kind": "port",
-            "sample": f"{int(n1)} packets of the {WORKLOAD} workload, FP64 transport, dSFMT, OpenMP ({dt1:.1f} s)"}
+            "sample": f"{int(n1)} packets of the {WORKLOAD} workload, FP64 transport, dSFMT, OpenMP ({dt1:.1f} s)",
+            "value_1thread": n2 / dt2, "sample_1thread": f"{int(n2)} packets, 1 thread ({dt2:.1f} s)"}
 
 
 def bench_heat(hip, steps_on=667, steps_off=1333):
