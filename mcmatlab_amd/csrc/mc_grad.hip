@@ -16,12 +16,19 @@
 //               exp(-tau), so d log p(path)/d log s = K - tau_total (K
 //               interactions, tau_total the optical depth traversed); the
 //               estimator is J_packet * (K - tau_total).
-// Each lane runs its packets twice with the same Philox stream: pass 1 finds
-// the packet's box deposit D, pass 2 replays the path and adds the dJ/dmua
-// terms with FP64 atomics (only packets with D > 0 are replayed).  The file
-// is built without FMA contraction so that both passes, compiled as separate
-// instantiations, evaluate identical operation sequences and retrace the
-// same path.
+// Each packet runs twice with the same Philox stream: pass 1 finds the
+// packet's box deposit D, pass 2 replays the path and adds the dJ/dmua terms
+// with FP64 atomics (only packets with D > 0 are replayed).  Both passes run
+// the same instructions (the pass is a runtime flag), so pass 2 retraces
+// pass 1's path exactly; the file is built without FMA contraction.
+//
+// Lanes are persistent and each packet is a small state machine like the
+// transport's: FETCH (take the next packet number, one atomic per wavefront,
+// and launch it), MOVE (one voxel sub-step), SCATTER (roulette +
+// deflection).  Each iteration the wavefront runs one state for the lanes in
+// it (MOVE while at least half of the live lanes want to move, otherwise the
+// state with more waiting lanes), so a wavefront never waits for its
+// longest-lived packet before starting others.
 #include <hip/hip_runtime.h>
 
 #include <cmath>
@@ -41,67 +48,22 @@ struct GradArgs {
   double* grad;         // [L] sum over packets of dJ_packet/dmua
   double* sums;         // [4] sum D, sum D^2, sum D*S, sum (D*S)^2
   unsigned* aborted;
+  unsigned long long* next;  // next packet number to hand out
 };
 
-// One packet; pass 1 returns its box deposit and score, pass 2 (given the
-// deposit D) adds the packet's dJ/dmua terms.
-template <bool FRESNEL, bool PASS2>
-__device__ inline bool grad_packet(const McParams& p, const GradArgs& g, const Medium* med, const uint8_t* cdf, PacketRng& R, uint64_t k,
-                                   double& D, double& score) {
+// Launch packet k of the stream (both passes start here).
+MCM_FN bool grad_begin(Photon& P, const McParams& p, const GradArgs& g, const Medium* med, const uint8_t* cdf, PacketRng& R,
+                       uint64_t k) {
   R.start(k);
-  Photon P;
   if (!launch(P, p, med, R)) return false;
   new_voxel_properties(P, p, med, cdf);
   if (g.muaMap) P.mua = g.muaMap[P.j];
-  double tau = 0.0;
-  uint32_t K = 0;
-  double rest = D;
-  double acc = 0.0;
-  int accJ = -1;
-  while (P.alive) {
-    while (P.alive && P.stepLeft > 0.f) {
-      if (P.substeps >= kMaxSubsteps) {
-        P.alive = false;
-        break;
-      }
-      const int j = P.j;
-      const bool inside = P.insideVolume;
-      const float mus = P.mus;
-      const int ix = clamp_axis(P.i[0], p.n[0]), iy = clamp_axis(P.i[1], p.n[1]), iz = clamp_axis(P.i[2], p.n[2]);
-      const bool boxed = inside && ix >= g.box[0] && ix < g.box[1] && iy >= g.box[2] && iy < g.box[3] && iz >= g.box[4] && iz < g.box[5];
-      const MoveResult mr = move<FRESNEL>(P, p, med);
-      if (FRESNEL && mr.fresnel) fresnel(P, p, med, mr.jNew, mr.mNew, R);
-      tau += (double)mr.s * (double)mus;
-      if (!PASS2) {
-        if (boxed) D += (double)mr.absorb;
-      } else {
-        if (boxed) rest -= (double)mr.absorb;
-        double gv = -(double)mr.s * rest;
-        if (boxed) gv += (double)P.weight * (double)mr.s;
-        if (j == accJ) {
-          acc += gv;
-        } else {
-          if (accJ >= 0) unsafeAtomicAdd(&g.grad[accJ], acc);
-          accJ = j;
-          acc = gv;
-        }
-      }
-      if (!P.sameVoxel) {
-        check_escape(P, p);
-        if (P.alive) {
-          new_voxel_properties(P, p, med, cdf);
-          if (g.muaMap) P.mua = g.muaMap[P.j];
-        }
-      }
-    }
-    if (!P.alive) break;
-    ++K;  // the flight ended in an interaction
-    roulette(P, R);
-    if (P.alive) scatter(P, p, p.cdfs, 0, R);
-  }
-  if (PASS2 && accJ >= 0) unsafeAtomicAdd(&g.grad[accJ], acc);
-  score = (double)K - tau;
   return true;
+}
+
+__device__ inline unsigned long long grad_bcast64(unsigned long long v, int src) {
+  const unsigned lo = __shfl((unsigned)v, src), hi = __shfl((unsigned)(v >> 32), src);
+  return ((unsigned long long)hi << 32) | lo;
 }
 
 template <bool FRESNEL>
@@ -114,23 +76,126 @@ __global__ void __launch_bounds__(256) mc_grad(McParams p, GradArgs g) {
     sCdf[k] = p.cdfIdx[k];
   }
   __syncthreads();
+  constexpr int G_FETCH = 0, G_MOVE = 1, G_SCATTER = 2, G_DONE = 3;
+  const int lane = __lane_id();
   PacketRng R;
   R.key0 = g.key0;
   R.key1 = g.key1;
   R.stream = g.stream;
+  Photon P;
+  int st = G_FETCH;
+  bool pass2 = false;
+  uint64_t k = 0;
+  double D = 0.0, tau = 0.0, rest = 0.0, acc = 0.0;
+  uint32_t K = 0;
+  int accJ = -1;
   double s0 = 0, s1 = 0, s2 = 0, s3 = 0;
-  const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
-  for (uint64_t k = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; k < g.nPackets; k += stride) {
-    double D = 0.0, S = 0.0;
-    if (!grad_packet<FRESNEL, false>(p, g, sMed, sCdf, R, k, D, S)) {
-      atomicOr(g.aborted, 1u);
-      break;
+  for (;;) {
+    const unsigned long long bFetch = __ballot(st == G_FETCH);
+    const unsigned long long bMove = __ballot(st == G_MOVE);
+    const unsigned long long bScat = __ballot(st == G_SCATTER);
+    const int cF = __popcll(bFetch), cM = __popcll(bMove), cS = __popcll(bScat);
+    const int active = cF + cM + cS;
+    if (active == 0) break;
+    int phase = G_MOVE;
+    if (2 * cM < active) phase = cS >= cF ? G_SCATTER : G_FETCH;
+    bool ended = false;  // the lane's current pass ended this iteration
+    if (phase == G_FETCH) {
+      const int leader = __ffsll((long long)bFetch) - 1;
+      unsigned long long base = 0;
+      if (lane == leader) {
+        base = atomicAdd(g.next, (unsigned long long)cF);
+        if (atomicAdd(g.aborted, 0u)) base = g.nPackets;  // a failed launch stops the run
+      }
+      base = grad_bcast64(base, leader);
+      if (st == G_FETCH) {
+        k = base + (unsigned long long)__popcll(bFetch & ((1ull << lane) - 1ull));
+        if (k >= g.nPackets) {
+          st = G_DONE;
+        } else if (!grad_begin(P, p, g, sMed, sCdf, R, k)) {
+          atomicOr(g.aborted, 1u);
+          st = G_DONE;
+        } else {
+          st = G_MOVE;
+          pass2 = false;
+          D = 0.0;
+          tau = 0.0;
+          K = 0;
+        }
+      }
+    } else if (phase == G_MOVE) {
+      if (st == G_MOVE) {
+        if (P.alive && P.stepLeft > 0.f) {
+          if (P.substeps >= kMaxSubsteps) {
+            P.alive = false;
+          } else {
+            const int j = P.j;
+            const bool inside = P.insideVolume;
+            const float mus = P.mus;
+            const int ix = clamp_axis(P.i[0], p.n[0]), iy = clamp_axis(P.i[1], p.n[1]), iz = clamp_axis(P.i[2], p.n[2]);
+            const bool boxed = inside && ix >= g.box[0] && ix < g.box[1] && iy >= g.box[2] && iy < g.box[3] && iz >= g.box[4] &&
+                               iz < g.box[5];
+            const MoveResult mr = move<FRESNEL>(P, p, sMed);
+            if (FRESNEL && mr.fresnel) fresnel(P, p, sMed, mr.jNew, mr.mNew, R);
+            tau += (double)mr.s * (double)mus;
+            if (!pass2) {
+              if (boxed) D += (double)mr.absorb;
+            } else {
+              if (boxed) rest -= (double)mr.absorb;
+              double gv = -(double)mr.s * rest;
+              if (boxed) gv += (double)P.weight * (double)mr.s;
+              if (j == accJ) {
+                acc += gv;
+              } else {
+                if (accJ >= 0) unsafeAtomicAdd(&g.grad[accJ], acc);
+                accJ = j;
+                acc = gv;
+              }
+            }
+            if (!P.sameVoxel) {
+              check_escape(P, p);
+              if (P.alive) {
+                new_voxel_properties(P, p, sMed, sCdf);
+                if (g.muaMap) P.mua = g.muaMap[P.j];
+              }
+            }
+          }
+        }
+        if (!P.alive) ended = true;
+        else if (!(P.stepLeft > 0.f)) st = G_SCATTER;
+      }
+    } else {
+      if (st == G_SCATTER) {
+        ++K;  // the flight ended in an interaction
+        roulette(P, R);
+        if (P.alive) scatter(P, p, p.cdfs, 0, R);
+        if (P.alive) st = G_MOVE;
+        else ended = true;
+      }
     }
-    s0 += D;
-    s1 += D * D;
-    s2 += D * S;
-    s3 += (D * S) * (D * S);
-    if (D > 0.0) grad_packet<FRESNEL, true>(p, g, sMed, sCdf, R, k, D, S);
+    if (ended) {
+      if (!pass2) {
+        const double S = (double)K - tau;
+        s0 += D;
+        s1 += D * D;
+        s2 += D * S;
+        s3 += (D * S) * (D * S);
+        st = G_FETCH;
+        if (D > 0.0 && grad_begin(P, p, g, sMed, sCdf, R, k)) {
+          pass2 = true;
+          rest = D;
+          tau = 0.0;
+          K = 0;
+          acc = 0.0;
+          accJ = -1;
+          st = G_MOVE;
+        }
+      } else {
+        if (accJ >= 0) unsafeAtomicAdd(&g.grad[accJ], acc);
+        accJ = -1;
+        st = G_FETCH;
+      }
+    }
   }
   // wavefront then workgroup reduction of the four sums
   for (int off = 32; off > 0; off >>= 1) {
@@ -191,12 +256,15 @@ extern "C" int mcm_mc_grad_run(mcm_mc_plan* pl, const mcm_mc_grad_input* in, mcm
   g.grad = dGrad.as<double>();
   g.sums = dSums.as<double>();
   g.aborted = dAbort.as<unsigned>();
+  DevBuf dNext;
+  if ((rc = zeros(dNext, sizeof(unsigned long long), s))) return rc;
+  g.next = dNext.as<unsigned long long>();
   int per = 0;
   const void* kern = T.fresnelPossible ? (const void*)mc_grad<true> : (const void*)mc_grad<false>;
   if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per, kern, 256, 0) != hipSuccess || per < 1) per = 2;
   const uint64_t lanes = (uint64_t)per * cu_count(pl->device) * 256;
-  // no more lanes than packets / 4, so the static packet assignment stays balanced
-  const unsigned blocks = (unsigned)std::max<uint64_t>(1, std::min<uint64_t>(lanes, (in->nPhotons + 1023) / 4) / 256);
+  // persistent lanes; no more lanes than packets
+  const unsigned blocks = (unsigned)std::max<uint64_t>(1, std::min<uint64_t>(lanes, in->nPhotons + 255) / 256);
   if (T.fresnelPossible) hipLaunchKernelGGL(mc_grad<true>, dim3(blocks), dim3(256), 0, s, pl->prm, g);
   else hipLaunchKernelGGL(mc_grad<false>, dim3(blocks), dim3(256), 0, s, pl->prm, g);
   MCM_HIP(hipGetLastError());
