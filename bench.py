@@ -344,13 +344,11 @@ def bench_sweep(n_models=32, seed=3, heat_steps=1000, every=100, parts=("mc", "g
         total += t_group
     if "adjoint" in parts:
         t = time.perf_counter()
-        J = 0.0
-        for T0, Om0, P in cases:
-            r = api.heat_adjoint(T0, Om0, P, segments=[(heat_steps, P["dt"], True)], checkpointEvery=every)
-            J += r["J"]
+        rs = api.heat_adjoints([(T0, Om0, P, [(heat_steps, P["dt"], True)]) for T0, Om0, P in cases], workers=4,
+                               checkpointEvery=every)
         t_adj = time.perf_counter() - t
-        log(f"adjoints {t_adj:.2f}s")
-        res.update(adjoint_s=t_adj, damage_sum=J)
+        log(f"adjoints (4 at a time) {t_adj:.2f}s")
+        res.update(adjoint_s=t_adj, damage_sum=sum(r["J"] for r in rs))
         total += t_adj
     res["value"] = total
     return res

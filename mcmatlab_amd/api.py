@@ -230,6 +230,19 @@ def heat_adjoint(T, Omega, P, segments=None, targetMask=None, checkpointEvery=0,
     return {"J": out.J, "dJ_dQ": f(gq), "dJ_dT0": f(gt), "Omega": f(om)}
 
 
+def heat_adjoints(cases, workers=4, **kw):
+    """heat_adjoint of independent models, `workers` at a time: each call
+    runs on its own plan and stream from its own host thread (the library
+    releases the interpreter lock while it works), so small models overlap
+    on the GPU.  cases = [(T, Omega, P, segments), ...]; keyword arguments
+    go to every heat_adjoint call.  Returns the results in order."""
+    from concurrent.futures import ThreadPoolExecutor
+
+    with ThreadPoolExecutor(max_workers=max(1, int(workers))) as ex:
+        futs = [ex.submit(heat_adjoint, T, Om, P, segments=segs, **kw) for T, Om, P, segs in cases]
+        return [f.result() for f in futs]
+
+
 class MCPlan:
     """Device-resident Monte Carlo plan (mcm_mc_plan_*): upload once, launch
     many times.  Used by the benchmark and by sweeps."""

@@ -132,3 +132,18 @@ def test_adjoint_target_mask_and_checkpoint_period(api):
     c = api.heat_adjoint(T0, Om0, P, segments=segs, checkpointEvery=7, targetMask=half)
     assert 0 < c["J"] < a["J"]
     assert np.isclose(c["J"], float(np.sum(a["Omega"][half])), rtol=1e-12)
+
+
+def test_adjoints_in_threads_equal_one_at_a_time(api):
+    """heat_adjoints runs independent models from several host threads; each
+    result equals the same call made alone (same kernels, same order per
+    plan: bit for bit)."""
+    cases = []
+    for k, shape in enumerate([(14, 12, 10), (9, 16, 11), (12, 12, 12)]):
+        T0, Om0, P, segs = case(bt=k % 2, shape=shape, seed=10 + k)
+        cases.append((T0, Om0, P, segs))
+    alone = [api.heat_adjoint(T0, Om0, P, segments=segs, checkpointEvery=5) for T0, Om0, P, segs in cases]
+    many = api.heat_adjoints(cases, workers=3, checkpointEvery=5)
+    for a, b in zip(alone, many):
+        assert a["J"] == b["J"]
+        assert np.array_equal(a["dJ_dQ"], b["dJ_dQ"]) and np.array_equal(a["dJ_dT0"], b["dJ_dT0"])
