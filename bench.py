@@ -273,9 +273,10 @@ def bench_sweep(n_models=32, seed=3, heat_steps=1000, every=100, parts=("mc", "g
     grids (multi_grid_sweep: sides 64..247 from seed 3, cyclic x/y, 1e6-1e7
     packets each): forward MC of every model as one multi-grid launch
     sequence (and, for comparison, as plain launches one model after
-    another), the absorbed-power gradient of each model, 1000 heat steps of
-    every model as one plan group (and one plan after another), and each
-    model's heat adjoint.  `parts` selects stages (the heat and adjoint
+    another), the absorbed-power gradient of each model (four at a time from
+    host threads, and one after another), 1000 heat steps of every model as
+    one plan group (and one plan after another), and each model's heat
+    adjoint (four at a time).  `parts` selects stages (the heat and adjoint
     stages need "mc" for their heating rates).  Stages print progress to
     stderr."""
     from mcmatlab_amd import api
@@ -312,8 +313,13 @@ def bench_sweep(n_models=32, seed=3, heat_steps=1000, every=100, parts=("mc", "g
         t = time.perf_counter()
         for p, w in zip(plans, W):
             p.absorbed_power(w["box"], w["nPhotons"], seed=5)
+        t_grad1 = time.perf_counter() - t
+        log(f"gradients one after another {t_grad1:.2f}s")
+        t = time.perf_counter()
+        api.absorbed_powers([(p, {"box": w["box"], "nPhotons": w["nPhotons"], "seed": 5}) for p, w in zip(plans, W)], workers=4)
         t_grad = time.perf_counter() - t
-        log(f"gradients {t_grad:.2f}s")
+        log(f"gradients (4 at a time) {t_grad:.2f}s")
+        res["grad_serial_s"] = t_grad1
         res.update(grad_s=t_grad, grad_packets_per_s=packets / t_grad)
         total += t_grad
     for p in plans:

@@ -243,6 +243,18 @@ def heat_adjoints(cases, workers=4, **kw):
         return [f.result() for f in futs]
 
 
+def absorbed_powers(jobs, workers=4):
+    """MCPlan.absorbed_power of independent plans, `workers` at a time from
+    host threads (each plan has its own stream, so the runs overlap on the
+    GPU and one model's long-lived packets do not idle it).  jobs =
+    [(plan, kwargs), ...]; returns the results in order."""
+    from concurrent.futures import ThreadPoolExecutor
+
+    with ThreadPoolExecutor(max_workers=max(1, int(workers))) as ex:
+        futs = [ex.submit(lambda pk: pk[0].absorbed_power(**pk[1]), job) for job in jobs]
+        return [f.result() for f in futs]
+
+
 class MCPlan:
     """Device-resident Monte Carlo plan (mcm_mc_plan_*): upload once, launch
     many times.  Used by the benchmark and by sweeps."""

@@ -151,3 +151,27 @@ def test_grad_torch_autograd(api, blobs):
     r = plan.absorbed_power(box, 20000, seed=9, muaMap=mua)
     assert math.isclose(float(J), r["J"], rel_tol=1e-9)
     np.testing.assert_allclose(t.grad.numpy(), r["dJ_dmua"], rtol=1e-5, atol=1e-12)
+
+
+def test_gradients_in_threads_equal_one_at_a_time(api):
+    """absorbed_powers runs independent plans from several host threads; each
+    result equals the same call made alone (up to the order of FP64 atomic
+    sums)."""
+    from mcmatlab_amd.workloads import blob_absorber_mc
+
+    jobs, plans = [], []
+    for n in (24, 32, 28):
+        m, mua, box = blob_absorber_mc(n=n, seed=n)
+        p = api.MCPlan(m)
+        plans.append(p)
+        jobs.append((p, {"box": box, "nPhotons": 20000, "seed": 9, "muaMap": mua}))
+    try:
+        alone = [p.absorbed_power(**kw) for p, kw in jobs]
+        many = api.absorbed_powers(jobs, workers=3)
+    finally:
+        for p in plans:
+            p.close()
+    for a, b in zip(alone, many):
+        assert math.isclose(a["J"], b["J"], rel_tol=1e-12)
+        assert math.isclose(a["dJ_dlogmus"], b["dJ_dlogmus"], rel_tol=1e-12, abs_tol=1e-15)
+        np.testing.assert_allclose(b["dJ_dmua"], a["dJ_dmua"], rtol=1e-9, atol=1e-12 * float(np.max(np.abs(a["dJ_dmua"]))))
