@@ -828,7 +828,7 @@ __global__ void heat_adj_damage(HeatArgs<double> a, const double* __restrict__ x
 // `beta` holds the eliminated diagonal.
 template <int AXIS>
 __global__ void __launch_bounds__(kHeatTPB) heat_adj_solve(HeatArgs<double> a, const double* __restrict__ r, double* y, double* beta) {
-  const int nx = a.nx, ny = a.ny, nz = a.nz, nxy = nx * ny, nM = a.nM;
+  const int nx = a.nx, ny = a.ny, nz = a.nz, nxy = nx * ny;
   const int line = blockIdx.x * blockDim.x + threadIdx.x;
   int i0, st, n;
   bool lf;
@@ -1068,7 +1068,7 @@ __global__ void __launch_bounds__(kHeatTPB) heat_adj_combine(HeatArgs<double> a,
                                                             const double* __restrict__ mu3, const double* __restrict__ x0,
                                                             const double* __restrict__ x1, const uint8_t* __restrict__ target,
                                                             double* lamOut, double* dQ, int lightsOn) {
-  const int nx = a.nx, ny = a.ny, nz = a.nz, nxy = nx * ny, nM = a.nM;
+  const int nx = a.nx, ny = a.ny, nz = a.nz, nxy = nx * ny;
   const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= (int64_t)nxy * nz) return;
   const int x = (int)(i % nx), y = (int)((i / nx) % ny), z = (int)(i / nxy);

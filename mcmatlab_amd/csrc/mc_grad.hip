@@ -240,25 +240,30 @@ extern "C" int mcm_mc_grad_run(mcm_mc_plan* pl, const mcm_mc_grad_input* in, mcm
   int rc = select_wavelength(pl, in->iL);
   if (rc) return rc;
   hipStream_t s = pl->stream;
-  DevBuf dMap, dGrad, dSums, dAbort;
-  if (in->muaMap && (rc = upload(dMap, in->muaMap, (size_t)T.L, s))) return rc;
-  if ((rc = zeros(dGrad, (size_t)T.L * sizeof(double), s)) || (rc = zeros(dSums, 4 * sizeof(double), s)) ||
-      (rc = zeros(dAbort, sizeof(unsigned), s)))
-    return rc;
+  // device buffers live in the plan across calls: [L] floats for the map,
+  // [L] doubles for the gradient, and 64 bytes for the four sums, the abort
+  // flag and the packet counter
+  if (!pl->gradOut.p) {
+    MCM_HIP(hipMalloc(&pl->gradMap.p, (size_t)T.L * sizeof(float)));
+    MCM_HIP(hipMalloc(&pl->gradOut.p, (size_t)T.L * sizeof(double)));
+    MCM_HIP(hipMalloc(&pl->gradSmall.p, 64));
+  }
+  if (in->muaMap) MCM_HIP(hipMemcpyAsync(pl->gradMap.p, in->muaMap, (size_t)T.L * sizeof(float), hipMemcpyHostToDevice, s));
+  MCM_HIP(hipMemsetAsync(pl->gradOut.p, 0, (size_t)T.L * sizeof(double), s));
+  MCM_HIP(hipMemsetAsync(pl->gradSmall.p, 0, 64, s));
+  char* small = (char*)pl->gradSmall.p;
   GradArgs g;
   std::memset(&g, 0, sizeof g);
-  g.muaMap = in->muaMap ? dMap.as<float>() : nullptr;
+  g.muaMap = in->muaMap ? pl->gradMap.as<float>() : nullptr;
   for (int a = 0; a < 6; ++a) g.box[a] = (int)in->box[a];
   g.nPackets = in->nPhotons;
   g.key0 = (uint32_t)in->seed;
   g.key1 = (uint32_t)(in->seed >> 32);
   g.stream = (uint32_t)in->iL;
-  g.grad = dGrad.as<double>();
-  g.sums = dSums.as<double>();
-  g.aborted = dAbort.as<unsigned>();
-  DevBuf dNext;
-  if ((rc = zeros(dNext, sizeof(unsigned long long), s))) return rc;
-  g.next = dNext.as<unsigned long long>();
+  g.grad = pl->gradOut.as<double>();
+  g.sums = (double*)small;                          // bytes 0..31
+  g.next = (unsigned long long*)(small + 32);       // bytes 32..39
+  g.aborted = (unsigned*)(small + 40);              // bytes 40..43
   int per = 0;
   const void* kern = T.fresnelPossible ? (const void*)mc_grad<true> : (const void*)mc_grad<false>;
   if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per, kern, 256, 0) != hipSuccess || per < 1) per = 2;
@@ -268,11 +273,14 @@ extern "C" int mcm_mc_grad_run(mcm_mc_plan* pl, const mcm_mc_grad_input* in, mcm
   if (T.fresnelPossible) hipLaunchKernelGGL(mc_grad<true>, dim3(blocks), dim3(256), 0, s, pl->prm, g);
   else hipLaunchKernelGGL(mc_grad<false>, dim3(blocks), dim3(256), 0, s, pl->prm, g);
   MCM_HIP(hipGetLastError());
+  unsigned char hs[64];
+  MCM_HIP(hipMemcpyAsync(hs, small, 64, hipMemcpyDeviceToHost, s));
+  if (out->dJ_dmua) MCM_HIP(hipMemcpyAsync(out->dJ_dmua, pl->gradOut.p, (size_t)T.L * sizeof(double), hipMemcpyDeviceToHost, s));
+  MCM_HIP(hipStreamSynchronize(s));
   double sums[4];
   unsigned aborted = 0;
-  MCM_HIP(hipMemcpyAsync(sums, dSums.p, sizeof sums, hipMemcpyDeviceToHost, s));
-  MCM_HIP(hipMemcpyAsync(&aborted, dAbort.p, sizeof aborted, hipMemcpyDeviceToHost, s));
-  MCM_HIP(hipStreamSynchronize(s));
+  std::memcpy(sums, hs, sizeof sums);
+  std::memcpy(&aborted, hs + 40, sizeof aborted);
   if (aborted) return fail(MCM_ERR_INVALID_ARGUMENT, "all packets launch outside the simulation cuboid");
   const double N = (double)in->nPhotons;
   const double power = T.spectrum[(size_t)in->iL];
@@ -281,10 +289,7 @@ extern "C" int mcm_mc_grad_run(mcm_mc_plan* pl, const mcm_mc_grad_input* in, mcm
   out->J_se = se(sums[0], sums[1]);
   out->dJ_dlogmus = power * sums[2] / N;
   out->dJ_dlogmus_se = se(sums[2], sums[3]);
-  if (out->dJ_dmua) {
-    std::vector<double> h((size_t)T.L);
-    MCM_HIP(hipMemcpy(h.data(), dGrad.p, h.size() * sizeof(double), hipMemcpyDeviceToHost));
-    for (int64_t k = 0; k < T.L; ++k) out->dJ_dmua[k] = power * h[k] / N;
-  }
+  if (out->dJ_dmua)
+    for (int64_t k = 0; k < T.L; ++k) out->dJ_dmua[k] = power * out->dJ_dmua[k] / N;
   return MCM_OK;
 }

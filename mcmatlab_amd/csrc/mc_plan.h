@@ -53,6 +53,7 @@ struct mcm_mc_plan {
   DevBuf M, normals, media, cdfIdx, cdfs, fpid1, aid1, fpid2, aid2, S, muaNorm;
   DevBuf NFR, image, FF, NI[6], counters, nfrOut;
   DevBuf launchRec, escapeRec;  // per-launch packet records (no deposition criteria)
+  DevBuf gradMap, gradOut, gradSmall;  // gradient runs: mua map [L], dJ/dmua [L], sums/flags/counter
   uint64_t recCap = 0;
   std::vector<double> Shost;   // cumulative 3D source table of the current wavelength
   const float* sourceDistribution = nullptr;
