@@ -148,15 +148,16 @@ def cpu_baseline(model, seconds=15.0, seconds_1thread=5.0):
             "value_1thread": n2 / dt2, "sample_1thread": f"{int(n2)} packets, 1 thread ({dt2:.1f} s)"}
 
 
-def bench_heat(hip, steps_on=667, steps_off=1333):
-    """Config-2 heat schedule on the same 200^3 grid through the heat plan;
-    returns voxel-steps/s or None when the heat kernels are not built."""
+def bench_heat(hip, steps_on=667, steps_off=1333, scheme="dgadi"):
+    """Config-2 heat schedule on the same 200^3 grid through the heat plan
+    (FP64; scheme "dgadi", the reference's Douglas-Gunn ADI, or "explicit",
+    forward Euler, stable here at dt = 0.18 dtmax)."""
     from mcmatlab_amd import abi
     from mcmatlab_amd._lib import check, lib
     from mcmatlab_amd.workloads import layered_tissue_heat
 
     T0, Om0, Pon, Poff = layered_tissue_heat(n=200, steps_on=steps_on, steps_off=steps_off)
-    inp = abi.HeatInputs(T0, Om0, Pon)
+    inp = abi.HeatInputs(T0, Om0, Pon, scheme=abi.MCM_HEAT_EXPLICIT if scheme == "explicit" else abi.MCM_HEAT_DGADI)
     h = C.c_void_p()
     check(lib().mcm_heat_plan_create(C.byref(inp.struct), C.byref(h)))
     try:
@@ -171,9 +172,10 @@ def bench_heat(hip, steps_on=667, steps_off=1333):
     finally:
         lib().mcm_heat_plan_destroy(h)
     L = 200 ** 3
+    gbs = L * (steps_on * 41 + steps_off * 33) / dt / 1e9
     return {"voxel_steps_per_s": L * (steps_on + steps_off) / dt, "steps": steps_on + steps_off, "seconds": dt,
-            "scheme": "DG-ADI", "dtype": "f64",
-            "algorithmic_GBs": L * (steps_on * 41 + steps_off * 33) / dt / 1e9}
+            "scheme": "explicit" if scheme == "explicit" else "DG-ADI", "dtype": "f64", "algorithmic_GBs": gbs,
+            "frac": gbs / HBM_PEAK_GBS}
 
 
 def bench_spectral(seed=7):
@@ -218,17 +220,16 @@ def bench_spectral(seed=7):
             "absorbed_fraction_first_last": [absorbed[0], absorbed[-1]]}
 
 
-def bench_heat_line():
-    """The heat propagator alone: 667 lights-on + 1333 lights-off FP64 DG-ADI
-    steps on the 200^3 grid, against the compulsory traffic of 41 / 33 bytes
-    per voxel-step."""
-    r = bench_heat(Hip())
-    L = 200 ** 3
-    alg = L * (667 * 41 + 1333 * 33)
-    gbs = alg / r["seconds"] / 1e9
-    return {"metric": "voxel-steps/s, 200^3 heat (FP64 DG-ADI, 667 on + 1333 off)", "value": r["voxel_steps_per_s"],
+def bench_heat_line(scheme="dgadi"):
+    """The heat propagator alone: 667 lights-on + 1333 lights-off FP64 steps
+    on the 200^3 grid, against the compulsory traffic of 41 / 33 bytes per
+    voxel-step."""
+    r = bench_heat(Hip(), scheme=scheme)
+    name = "explicit" if scheme == "explicit" else "DG-ADI"
+    return {"metric": f"voxel-steps/s, 200^3 heat (FP64 {name}, 667 on + 1333 off)", "value": r["voxel_steps_per_s"],
             "unit": "voxel-steps/s", "ms_per_step": r["seconds"] / r["steps"] * 1e3, "dtype": "f64",
-            "roofline": {"bound": "hbm", "achieved": gbs, "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": gbs / HBM_PEAK_GBS}}
+            "roofline": {"bound": "hbm", "achieved": r["algorithmic_GBs"], "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                         "frac": r["frac"]}}
 
 
 def bench_grad(n=128, packets=10_000_000, seed=3):
@@ -369,9 +370,11 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-heat", action="store_true")
     ap.add_argument("--workload", default="layered_tissue_200_fresnel_tophat",
-                    choices=["layered_tissue_200_fresnel_tophat", "spectral64", "heat200", "grad128", "adjoint128", "sweep32"],
+                    choices=["layered_tissue_200_fresnel_tophat", "spectral64", "heat200", "heat200explicit", "grad128", "adjoint128",
+                             "sweep32"],
                     help="secondary workloads (each prints its own line): spectral64 batched vs plain launches of the "
-                         "64-wavelength workload; heat200 the heat propagator alone; grad128 absorbed-power gradients; "
+                         "64-wavelength workload; heat200 (heat200explicit) the DG-ADI (explicit) heat propagator alone; "
+                         "grad128 absorbed-power gradients; "
                          "adjoint128 the heat adjoint; sweep32 the 32-grid forward + backward sweep")
     ap.add_argument("--copies", type=int, default=0, choices=[0, 1, 2, 4, 8],
                     help="force the number of XCD-private fluence copies (0: the autotuner's choice)")
@@ -384,8 +387,8 @@ def main():
         print("bench.py measures one GPU only: this build has no multi-GPU path (run with --gpus 1)", file=sys.stderr)
         sys.exit(2)
 
-    secondary = {"spectral64": bench_spectral, "heat200": bench_heat_line, "grad128": bench_grad, "adjoint128": bench_adjoint,
-                 "sweep32": bench_sweep}
+    secondary = {"spectral64": bench_spectral, "heat200": bench_heat_line, "heat200explicit": lambda: bench_heat_line("explicit"),
+                 "grad128": bench_grad, "adjoint128": bench_adjoint, "sweep32": bench_sweep}
     if args.workload in secondary:
         kw = {"parts": tuple(args.sweep_parts.split(","))} if args.workload == "sweep32" else {}
         print(json.dumps(secondary[args.workload](**kw)), flush=True)
@@ -472,6 +475,7 @@ def main():
     if not args.no_heat:
         try:
             line["heat"] = bench_heat(hip)
+            line["heat_explicit"] = bench_heat(hip, scheme="explicit")
         except Exception as e:  # heat kernels not available in this build
             line["heat"] = {"error": str(e)[:200]}
     if not args.no_cpu:

@@ -703,39 +703,84 @@ __global__ void __launch_bounds__(kHeatTPB) heat_sweep_yz_tile(HeatArgs<double> 
 // sequential lane-per-line kernel)
 inline int wave_rows(int n) { return n <= 64 ? 1 : n <= 128 ? 2 : n <= 256 ? 4 : n <= 512 ? 8 : 0; }
 
-constexpr int kExplicitZ = 8;
+// Forward Euler step (no reference counterpart; the 7-point right-hand side
+// of substep 1a with all three axes fully weighted, the damage update and
+// the per-medium maxima of the DG-ADI step).  A workgroup owns a
+// kTileX x kTileY column tile and a run of kZRun z planes, marching up in z:
+// each plane of T and M (with its x/y halo) is staged through LDS once, so
+// every voxel is loaded from memory once and its four in-plane neighbours
+// come from LDS; the z neighbours are carried in registers.
+constexpr int kTileX = 32, kTileY = 8, kZRun = 16;
 
-// Forward Euler step (no reference counterpart; same neighbour order and
-// damage/maximum update as the DG-ADI step): lane per (x, y, chunk of 8 z),
-// lanes along x, z neighbours carried in registers along the chunk.
 template <typename RT>
-__global__ void __launch_bounds__(kHeatTPB) heat_explicit(HeatArgs<RT> a) {
+__global__ void __launch_bounds__(kTileX * kTileY) heat_explicit_tile(HeatArgs<RT> a) {
   __shared__ HeatLds<RT> L;
+  __shared__ RT sT[kTileY + 2][kTileX + 2];
+  __shared__ uint8_t sM[kTileY + 2][kTileX + 2];
   const RT* cT = stage_tables(a, L);
   const int nx = a.nx, ny = a.ny, nz = a.nz, nxy = nx * ny, nM = a.nM;
-  const int nzc = (nz + kExplicitZ - 1) / kExplicitZ;
-  const int lane = blockIdx.x * blockDim.x + threadIdx.x;
-  if (lane < nxy * nzc) {
-    const int col = lane % nxy, zc = lane / nxy;
-    const int x = col % nx, y = col / nx;
-    const int z0 = zc * kExplicitZ;
-    const RT dt = a.dt;
-    MaxRun run;
-    RT tBelow = z0 > 0 ? a.T1[col + (z0 - 1) * nxy] : (RT)0;
-    int mBelow = z0 > 0 ? (int)a.M[col + (z0 - 1) * nxy] : 0;
-    RT tHere = a.T1[col + z0 * nxy];
-    int mHere = a.M[col + z0 * nxy];
-    for (int z = z0; z < z0 + kExplicitZ && z < nz; ++z) {
-      const int i = col + z * nxy;
-      const RT tAbove = z != nz - 1 ? a.T1[i + nxy] : (RT)0;
-      const int mAbove = z != nz - 1 ? (int)a.M[i + nxy] : 0;
+  const int ntx = (nx + kTileX - 1) / kTileX, nty = (ny + kTileY - 1) / kTileY;
+  const int b = blockIdx.x;
+  const int x0 = (b % ntx) * kTileX, y0 = (b / ntx % nty) * kTileY, z0 = b / (ntx * nty) * kZRun;
+  const int tx = threadIdx.x % kTileX, ty = threadIdx.x / kTileX;
+  const int x = x0 + tx, y = y0 + ty;
+  const bool ok = x < nx && y < ny;
+  const int col = ok ? x + y * nx : 0;
+  const int zEnd = min(z0 + kZRun, nz);
+  // halo duty: the tile's edge threads also stage the neighbouring column
+  const int hx = tx == 0 ? x - 1 : (tx == kTileX - 1 ? x + 1 : -1);
+  const int hy = ty == 0 ? y - 1 : (ty == kTileY - 1 ? y + 1 : -1);
+  const bool hxOk = ok && hx >= 0 && hx < nx, hyOk = ok && hy >= 0 && hy < ny;
+  const int hxCol = hxOk ? hx + y * nx : 0, hyCol = hyOk ? x + hy * nx : 0;
+  const int sxh = tx == 0 ? 0 : kTileX + 1, syh = ty == 0 ? 0 : kTileY + 1;
+  const RT dt = a.dt;
+  MaxRun run;
+  RT tBelow = 0, tHere = 0;
+  int mBelow = 0, mHere = 0;
+  if (ok) {
+    tBelow = z0 > 0 ? a.T1[col + (z0 - 1) * nxy] : (RT)0;
+    mBelow = z0 > 0 ? (int)a.M[col + (z0 - 1) * nxy] : 0;
+    tHere = a.T1[col + z0 * nxy];
+    mHere = a.M[col + z0 * nxy];
+  }
+  for (int z = z0; z < zEnd; ++z) {
+    const int i = col + z * nxy;
+    // this thread's loads for the step, all issued before the LDS stores
+    RT tAbove = 0, hxT = 0, hyT = 0;
+    int mAbove = 0, hxM = 0, hyM = 0;
+    if (ok && z != nz - 1) {
+      tAbove = a.T1[i + nxy];
+      mAbove = a.M[i + nxy];
+    }
+    if (hxOk) {
+      hxT = a.T1[hxCol + z * nxy];
+      hxM = a.M[hxCol + z * nxy];
+    }
+    if (hyOk) {
+      hyT = a.T1[hyCol + z * nxy];
+      hyM = a.M[hyCol + z * nxy];
+    }
+    const RT q = ok && a.lightsOn ? (RT)a.Q[i] : (RT)0;
+    __syncthreads();  // the previous plane's readers are done
+    sT[ty + 1][tx + 1] = tHere;
+    sM[ty + 1][tx + 1] = (uint8_t)mHere;
+    if (hxOk) {
+      sT[ty + 1][sxh] = hxT;
+      sM[ty + 1][sxh] = (uint8_t)hxM;
+    }
+    if (hyOk) {
+      sT[syh][tx + 1] = hyT;
+      sM[syh][tx + 1] = (uint8_t)hyM;
+    }
+    __syncthreads();
+    if (ok) {
       RT dTdt = 0;
       if (!frozen(a, x, y, z)) {
-        dTdt = a.lightsOn ? (RT)a.Q[i] : (RT)0;
-        if (x != 0) dTdt += (a.T1[i - 1] - tHere) * coef(cT, nM, mHere, a.M[i - 1], 0);
-        if (x != nx - 1) dTdt += (a.T1[i + 1] - tHere) * coef(cT, nM, mHere, a.M[i + 1], 0);
-        if (y != 0) dTdt += (a.T1[i - nx] - tHere) * coef(cT, nM, mHere, a.M[i - nx], 1);
-        if (y != ny - 1) dTdt += (a.T1[i + nx] - tHere) * coef(cT, nM, mHere, a.M[i + nx], 1);
+        dTdt = a.lightsOn ? q : (RT)0;
+        if (x != 0) dTdt += (sT[ty + 1][tx] - tHere) * coef(cT, nM, mHere, (int)sM[ty + 1][tx], 0);
+        if (x != nx - 1) dTdt += (sT[ty + 1][tx + 2] - tHere) * coef(cT, nM, mHere, (int)sM[ty + 1][tx + 2], 0);
+        if (y != 0) dTdt += (sT[ty][tx + 1] - tHere) * coef(cT, nM, mHere, (int)sM[ty][tx + 1], 1);
+        if (y != ny - 1) dTdt += (sT[ty + 2][tx + 1] - tHere) * coef(cT, nM, mHere, (int)sM[ty + 2][tx + 1], 1);
         if (z != 0) dTdt += (tBelow - tHere) * coef(cT, nM, mHere, mBelow, 2);
         if (z != nz - 1) dTdt += (tAbove - tHere) * coef(cT, nM, mHere, mAbove, 2);
       }
@@ -744,13 +789,13 @@ __global__ void __launch_bounds__(kHeatTPB) heat_explicit(HeatArgs<RT> a) {
       if (a.calcDamage && L.E[mHere] != 0)
         a.Om[i] += dt * (RT)exp(L.logA[mHere] - L.E[mHere] / (a.Rgas * ((t + tHere) / 2 + a.czero)));
       run.add(mHere, (double)t, L.maxKey);
-      tBelow = tHere;
-      mBelow = mHere;
-      tHere = tAbove;
-      mHere = mAbove;
     }
-    run.flush(L.maxKey);
+    tBelow = tHere;
+    mBelow = mHere;
+    tHere = tAbove;
+    mHere = mAbove;
   }
+  run.flush(L.maxKey);
   fold_lds_max(L.maxKey, a.maxKey, a.nM);
 }
 
@@ -1291,8 +1336,9 @@ void enqueue_step(mcm_heat_plan* P, double dt, int lightsOn) {
   const int tpb = kHeatTPB;
   HeatArgs<RT> a = make_args<RT>(P, dt, lightsOn);
   if (P->explicitScheme) {
-    const unsigned ge = (unsigned)(((int64_t)P->nx * P->ny * ((P->nz + kExplicitZ - 1) / kExplicitZ) + tpb - 1) / tpb);
-    hipLaunchKernelGGL(heat_explicit<RT>, dim3(ge), dim3(tpb), 0, P->stream, a);
+    const unsigned ge = (unsigned)(((P->nx + kTileX - 1) / kTileX) * ((P->ny + kTileY - 1) / kTileY) * ((P->nz + kZRun - 1) / kZRun));
+    hipLaunchKernelGGL(heat_explicit_tile<RT>, dim3(ge), dim3(kTileX * kTileY), 0, P->stream, a);
+    (void)tpb;
   } else {
     launch_sweep_x<RT>(P, a, (unsigned)((P->ny * P->nz + tpb - 1) / tpb));
     launch_sweep_yz<RT>(P, a, (unsigned)((P->nx * P->nz + tpb - 1) / tpb), (unsigned)((P->nx * P->ny + tpb - 1) / tpb));
