@@ -15,12 +15,12 @@ line (see the keys below), including:
                 8 TB/s HBM peak;
   cpu_baseline  the FP64 CPU restatement (OpenMP, dSFMT) timed on this host's
                 cores on a bounded sample of the same workload.
-  heat          the heat propagator on the same grid (2000 FP64 steps), when
-                the heat kernels are built.
+  heat          the heat propagator on the same grid (2000 FP64 DG-ADI steps);
+  heat_explicit the same schedule with the FP64 explicit scheme.
 
-Usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--packets N]
+Usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--packets N] [--copies R]
        python bench.py --copy-probe BYTES   (plain copy kernel for counter scaling)
-       python bench.py --workload spectral64|heat200|grad128|adjoint128   (secondary lines)
+       python bench.py --workload spectral64|heat200|heat200explicit|grad128|adjoint128|sweep32   (secondary lines)
 """
 from __future__ import annotations
 
@@ -244,10 +244,15 @@ def bench_grad(n=128, packets=10_000_000, seed=3):
     t = time.perf_counter()
     r = plan.absorbed_power(box, packets, seed=seed, muaMap=mua)
     dt = time.perf_counter() - t
+    # the forward alone: J (and the one-pass likelihood-ratio dJ/dlog mus)
+    t = time.perf_counter()
+    f = plan.absorbed_power(box, packets, seed=seed, muaMap=mua, grad=False)
+    dt_fwd = time.perf_counter() - t
     plan.close()
     return {"metric": "packets/s, absorbed power + dJ/dmua (128^3 blob absorber, 16^3 box)", "value": packets / dt,
-            "unit": "packets/s", "seconds": dt, "J": r["J"], "J_rel_se": r["J_se"] / r["J"],
-            "dJ_dlogmus": r["dJ_dlogmus"], "dJ_dlogmus_rel_se": abs(r["dJ_dlogmus_se"] / r["dJ_dlogmus"])}
+            "unit": "packets/s", "seconds": dt, "forward_s": dt_fwd, "grad_over_forward": dt / dt_fwd, "J": r["J"],
+            "J_forward": f["J"], "J_rel_se": r["J_se"] / r["J"], "dJ_dlogmus": r["dJ_dlogmus"],
+            "dJ_dlogmus_rel_se": abs(r["dJ_dlogmus_se"] / r["dJ_dlogmus"])}
 
 
 def bench_adjoint(n=128, steps=3000, every=100):

@@ -273,7 +273,8 @@ typedef struct mcm_mc_grad_input {
 } mcm_mc_grad_input;
 typedef struct mcm_mc_grad_output {
   double J, J_se;                  /* estimate and its standard error */
-  double *dJ_dmua;                 /* caller-owned [nx,ny,nz] or NULL: dJ/dmua per voxel [W cm / W] */
+  double *dJ_dmua;                 /* caller-owned [nx,ny,nz] or NULL: dJ/dmua per voxel [W cm / W];
+                                      NULL skips the replay pass (J and dJ_dlogmus need one pass) */
   double dJ_dlogmus, dJ_dlogmus_se;
 } mcm_mc_grad_output;
 int mcm_mc_grad_run(mcm_mc_plan *plan, const mcm_mc_grad_input *in, mcm_mc_grad_output *out);

@@ -49,6 +49,7 @@ struct GradArgs {
   double* sums;         // [4] sum D, sum D^2, sum D*S, sum (D*S)^2
   unsigned* aborted;
   unsigned long long* next;  // next packet number to hand out
+  int replay;                // run pass 2 (dJ/dmua requested)
 };
 
 // Launch packet k of the stream (both passes start here).
@@ -181,7 +182,7 @@ __global__ void __launch_bounds__(256) mc_grad(McParams p, GradArgs g) {
         s2 += D * S;
         s3 += (D * S) * (D * S);
         st = G_FETCH;
-        if (D > 0.0 && grad_begin(P, p, g, sMed, sCdf, R, k)) {
+        if (D > 0.0 && g.replay && grad_begin(P, p, g, sMed, sCdf, R, k)) {
           pass2 = true;
           rest = D;
           tau = 0.0;
@@ -264,6 +265,7 @@ extern "C" int mcm_mc_grad_run(mcm_mc_plan* pl, const mcm_mc_grad_input* in, mcm
   g.sums = (double*)small;                          // bytes 0..31
   g.next = (unsigned long long*)(small + 32);       // bytes 32..39
   g.aborted = (unsigned*)(small + 40);              // bytes 40..43
+  g.replay = out->dJ_dmua != nullptr;              // J and dJ/dlog s need pass 1 only
   int per = 0;
   const void* kern = T.fresnelPossible ? (const void*)mc_grad<true> : (const void*)mc_grad<false>;
   if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per, kern, 256, 0) != hipSuccess || per < 1) per = 2;

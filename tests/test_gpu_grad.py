@@ -175,3 +175,20 @@ def test_gradients_in_threads_equal_one_at_a_time(api):
         assert math.isclose(a["J"], b["J"], rel_tol=1e-12)
         assert math.isclose(a["dJ_dlogmus"], b["dJ_dlogmus"], rel_tol=1e-12, abs_tol=1e-15)
         np.testing.assert_allclose(b["dJ_dmua"], a["dJ_dmua"], rtol=1e-9, atol=1e-12 * float(np.max(np.abs(a["dJ_dmua"]))))
+
+
+def test_forward_only_skips_replay_with_same_J(api):
+    """Without a requested mua gradient the replay pass is skipped; J and the
+    likelihood-ratio dJ/dlog mus come from the first pass and are unchanged."""
+    from mcmatlab_amd.workloads import blob_absorber_mc
+
+    m, mua, box = blob_absorber_mc(n=32, seed=4)
+    p = api.MCPlan(m)
+    try:
+        a = p.absorbed_power(box, 30000, seed=2, muaMap=mua)
+        b = p.absorbed_power(box, 30000, seed=2, muaMap=mua, grad=False)
+    finally:
+        p.close()
+    assert b["dJ_dmua"] is None
+    assert math.isclose(a["J"], b["J"], rel_tol=1e-12)
+    assert math.isclose(a["dJ_dlogmus"], b["dJ_dlogmus"], rel_tol=1e-12, abs_tol=1e-15)
