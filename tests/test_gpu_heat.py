@@ -62,6 +62,9 @@ CASES = {
     "homogeneous": dict(hetero=False),
     "odd_grid": dict(shape=(13, 9, 11)),
     "tall_grid": dict(shape=(5, 7, 16), bt=0),
+    # several x/y tiles and z runs of the explicit stencil, several tiles of the y/z sweeps
+    "multi_tile": dict(n=72, shape=(70, 19, 37), steps=5),
+    "multi_tile_insulated": dict(n=72, shape=(33, 17, 49), steps=4, bt=0),
 }
 
 
