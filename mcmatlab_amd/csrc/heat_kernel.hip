@@ -70,7 +70,38 @@ struct HeatArgs {
   RT* B;                 // Thomas scratch: forward-eliminated diagonal
   RT* Om;                // damage integral (calcDamage)
   unsigned long long* maxKey;  // [nM] order_key of the running maximum
+  // sensor record of T1 (the state before the step), written by the first
+  // workgroup of the step's first kernel: column sCol (+ *sColBase)
+  int nS;
+  const long long* sIdx;
+  const RT* sW;
+  double* sOut;
+  const long long* sColBase;
+  long long sCol;
 };
+
+// recordTempSensors (:652-663): trilinear interpolation of T into column
+// `col` of the sensor record, sensors s = first, first + step, ...
+template <typename RT>
+__device__ inline void sensors_at(const RT* T, int nx, int nxy, const long long* idx, const RT* w, int nS, double* out, long long col,
+                                  int first, int step) {
+  for (int s = first; s < nS; s += step) {
+    const long long k = idx[s];
+    const RT wx = w[s], wy = w[s + nS], wz = w[s + 2 * nS];
+    out[s + col * nS] = (double)((1 - wx) * (1 - wy) * (1 - wz) * T[k] + (1 - wx) * (1 - wy) * wz * T[k + nxy] +
+                                 (1 - wx) * wy * (1 - wz) * T[k + nx] + (1 - wx) * wy * wz * T[k + nx + nxy] +
+                                 wx * (1 - wy) * (1 - wz) * T[k + 1] + wx * (1 - wy) * wz * T[k + 1 + nxy] +
+                                 wx * wy * (1 - wz) * T[k + 1 + nx] + wx * wy * wz * T[k + 1 + nx + nxy]);
+  }
+}
+
+// The step kernels' share of the sensor record (first workgroup only).
+template <typename RT>
+__device__ inline void step_sensors(const HeatArgs<RT>& a) {
+  if (a.nS == 0 || blockIdx.x != 0) return;
+  const long long col = a.sCol + (a.sColBase ? *a.sColBase : 0);
+  sensors_at(a.T1, a.nx, a.nx * a.ny, a.sIdx, a.sW, a.nS, a.sOut, col, (int)threadIdx.x, (int)blockDim.x);
+}
 
 template <typename RT>
 __device__ inline bool frozen(const HeatArgs<RT>& a, int x, int y, int z) {
@@ -285,6 +316,7 @@ constexpr int kSweepU = 8;
 template <typename RT>
 __global__ void __launch_bounds__(kHeatTPB) heat_sweep_x(HeatArgs<RT> a) {
   __shared__ HeatLds<RT> L;
+  step_sensors(a);
   const RT* cT = stage_tables(a, L);
   const int line = blockIdx.x * blockDim.x + threadIdx.x;
   if (line >= a.ny * a.nz) return;
@@ -447,6 +479,7 @@ __device__ inline void wave_tridiag(const double (&a)[S], const double (&b)[S], 
 template <int S>
 __global__ void __launch_bounds__(kHeatTPB) heat_sweep_x_wave(HeatArgs<double> a) {
   __shared__ double sc[kLdsCoef];
+  step_sensors(a);
   const int nc = a.nM * a.nM * 3;
   if (nc <= kLdsCoef)
     for (int k = threadIdx.x; k < nc; k += blockDim.x) sc[k] = a.c[k];
@@ -717,6 +750,7 @@ __global__ void __launch_bounds__(kTileX * kTileY) heat_explicit_tile(HeatArgs<R
   __shared__ HeatLds<RT> L;
   __shared__ RT sT[kTileY + 2][kTileX + 2];
   __shared__ uint8_t sM[kTileY + 2][kTileX + 2];
+  step_sensors(a);
   const RT* cT = stage_tables(a, L);
   const int nx = a.nx, ny = a.ny, nz = a.nz, nxy = nx * ny, nM = a.nM;
   const int ntx = (nx + kTileX - 1) / kTileX, nty = (ny + kTileY - 1) / kTileY;
@@ -799,20 +833,11 @@ __global__ void __launch_bounds__(kTileX * kTileY) heat_explicit_tile(HeatArgs<R
   fold_lds_max(L.maxKey, a.maxKey, a.nM);
 }
 
-// recordTempSensors (:652-663): trilinear interpolation of the new T into
-// column `col` (plus *colBase when given) of the sensor record
+// Sensor record of T into column `col` on its own (the final state of a
+// run, recorded when the record is read).
 template <typename RT>
-__global__ void heat_sensors(const RT* T, int nx, int nxy, const long long* idx, const RT* w, int nS, double* out, const long long* colBase,
-                             long long col) {
-  if (colBase) col += *colBase;
-  for (int s = threadIdx.x; s < nS; s += blockDim.x) {
-    const long long k = idx[s];
-    const RT wx = w[s], wy = w[s + nS], wz = w[s + 2 * nS];
-    out[s + col * nS] = (double)((1 - wx) * (1 - wy) * (1 - wz) * T[k] + (1 - wx) * (1 - wy) * wz * T[k + nxy] +
-                                 (1 - wx) * wy * (1 - wz) * T[k + nx] + (1 - wx) * wy * wz * T[k + nx + nxy] +
-                                 wx * (1 - wy) * (1 - wz) * T[k + 1] + wx * (1 - wy) * wz * T[k + 1 + nxy] +
-                                 wx * wy * (1 - wz) * T[k + 1 + nx] + wx * wy * wz * T[k + 1 + nx + nxy]);
-  }
+__global__ void heat_sensors(const RT* T, int nx, int nxy, const long long* idx, const RT* w, int nS, double* out, long long col) {
+  sensors_at(T, nx, nxy, idx, w, nS, out, col, (int)threadIdx.x, (int)blockDim.x);
 }
 
 // step counter of graph-replayed steps: set before a series of replays,
@@ -1175,6 +1200,7 @@ struct mcm_heat_plan {
   int sinked = 0, calcDamage = 0;
   int64_t nS = 0;
   int64_t stepsDone = 0;
+  int64_t sensorsRecorded = 0;  // sensor columns 0 .. sensorsRecorded-1 are in `sensors`
   int64_t sensorCap = 0;      // columns allocated in `sensors`
   bool interrupted = false;
   int (*interrupt)(void) = nullptr;
@@ -1230,7 +1256,7 @@ int halloc(HeatBuf& b, size_t bytes) {
 
 template <typename RT>
 HeatArgs<RT> make_args(mcm_heat_plan* P, double dt, int lightsOn) {
-  HeatArgs<RT> a;
+  HeatArgs<RT> a{};
   a.nx = P->nx;
   a.ny = P->ny;
   a.nz = P->nz;
@@ -1273,8 +1299,7 @@ template <typename RT>
 int record_sensors(mcm_heat_plan* P, int64_t col) {
   if (P->nS == 0) return MCM_OK;
   hipLaunchKernelGGL(heat_sensors<RT>, dim3(1), dim3(256), 0, P->stream, (const RT*)P->T1.p, P->nx, P->nx * P->ny,
-                     (const long long*)P->sIdx.p, (const RT*)P->sW.p, (int)P->nS, (double*)P->sensors.p, (const long long*)nullptr,
-                     (long long)col);
+                     (const long long*)P->sIdx.p, (const RT*)P->sW.p, (int)P->nS, (double*)P->sensors.p, (long long)col);
   MCM_HIP(hipGetLastError());
   return MCM_OK;
 }
@@ -1332,9 +1357,18 @@ void launch_sweep_yz(mcm_heat_plan* P, const HeatArgs<RT>& a, unsigned gy, unsig
 
 // The kernels of one time step on the plan's stream; swaps the state buffers.
 template <typename RT>
-void enqueue_step(mcm_heat_plan* P, double dt, int lightsOn) {
+void enqueue_step(mcm_heat_plan* P, double dt, int lightsOn, const long long* colBase, long long col) {
   const int tpb = kHeatTPB;
   HeatArgs<RT> a = make_args<RT>(P, dt, lightsOn);
+  if (P->nS) {
+    // the step's first kernel records the state it starts from
+    a.nS = (int)P->nS;
+    a.sIdx = (const long long*)P->sIdx.p;
+    a.sW = (const RT*)P->sW.p;
+    a.sOut = (double*)P->sensors.p;
+    a.sColBase = colBase;
+    a.sCol = col;
+  }
   if (P->explicitScheme) {
     const unsigned ge = (unsigned)(((P->nx + kTileX - 1) / kTileX) * ((P->ny + kTileY - 1) / kTileY) * ((P->nz + kZRun - 1) / kZRun));
     hipLaunchKernelGGL(heat_explicit_tile<RT>, dim3(ge), dim3(kTileX * kTileY), 0, P->stream, a);
@@ -1347,17 +1381,11 @@ void enqueue_step(mcm_heat_plan* P, double dt, int lightsOn) {
 }
 
 // K (even) steps as graph nodes on the plan's stream, which is capturing:
-// sensor columns are taken from the device step counter, which the last
-// node advances by K.  An even K leaves T1/T2 where they were.
+// step k records sensor column (device step counter) + k, and the last node
+// advances the counter by K.  An even K leaves T1/T2 where they were.
 template <typename RT>
 void enqueue_graph_steps(mcm_heat_plan* P, int64_t K, double dt, int lightsOn) {
-  for (int64_t k = 0; k < K; ++k) {
-    enqueue_step<RT>(P, dt, lightsOn);
-    if (P->nS)
-      hipLaunchKernelGGL(heat_sensors<RT>, dim3(1), dim3(256), 0, P->stream, (const RT*)P->T1.p, P->nx, P->nx * P->ny,
-                         (const long long*)P->sIdx.p, (const RT*)P->sW.p, (int)P->nS, (double*)P->sensors.p,
-                         (const long long*)P->stepDev.p, (long long)(k + 1));
-  }
+  for (int64_t k = 0; k < K; ++k) enqueue_step<RT>(P, dt, lightsOn, (const long long*)P->stepDev.p, (long long)k);
   if (P->nS) hipLaunchKernelGGL(heat_step_counter, dim3(1), dim3(64), 0, P->stream, (long long*)P->stepDev.p, (long long)K, 1);
 }
 
@@ -1419,6 +1447,7 @@ int advance(mcm_heat_plan* P, int64_t steps, double dt, int lightsOn) {
       }
       MCM_HIP(hipGraphLaunch(g, P->stream));
       P->stepsDone += K;
+      P->sensorsRecorded = P->stepsDone;
     }
   }
   for (; n < steps; ++n) {
@@ -1426,10 +1455,10 @@ int advance(mcm_heat_plan* P, int64_t steps, double dt, int lightsOn) {
       P->interrupted = true;
       break;
     }
-    enqueue_step<RT>(P, dt, lightsOn);
+    enqueue_step<RT>(P, dt, lightsOn, nullptr, P->stepsDone);
     MCM_HIP(hipGetLastError());
     ++P->stepsDone;
-    if ((rc = record_sensors<RT>(P, P->stepsDone))) return rc;
+    P->sensorsRecorded = P->stepsDone;
   }
   return MCM_OK;
 }
@@ -1483,6 +1512,13 @@ int download(mcm_heat_plan* P, mcm_heat_output* out) {
     MCM_HIP(hipStreamSynchronize(P->stream));
   } else if (!P->calcDamage && out->Omega) {
     out->Omega[0] = NAN;
+  }
+  if (P->nS && P->sensorsRecorded <= P->stepsDone) {
+    // the current state's column (steps record the state they start from)
+    int rc = ensure_sensor_capacity(P, P->stepsDone + 1);
+    if (rc) return rc;
+    if ((rc = record_sensors<RT>(P, P->stepsDone))) return rc;
+    P->sensorsRecorded = P->stepsDone + 1;
   }
   if (out->sensorTemps && P->nS) {
     MCM_HIP(hipMemcpyAsync(out->sensorTemps, P->sensors.p, (size_t)((P->stepsDone + 1) * P->nS) * sizeof(double), hipMemcpyDeviceToHost,
@@ -1629,8 +1665,6 @@ int mcm_heat_plan_create(const mcm_heat_input* in, mcm_heat_plan** out) {
   rc = P->fp32 ? upload_state<float>(P.get(), in) : upload_state<double>(P.get(), in);
   if (rc) return rc;
   if ((rc = ensure_sensor_capacity(P.get(), std::max<int64_t>(in->steps, 1) + 1))) return rc;
-  rc = P->fp32 ? record_sensors<float>(P.get(), 0) : record_sensors<double>(P.get(), 0);
-  if (rc) return rc;
   MCM_HIP(hipStreamSynchronize(s));
   *out = P.release();
   return MCM_OK;
@@ -1739,7 +1773,10 @@ int mcm_heat_group_advance(mcm_heat_group* G, int64_t steps, const double* dt, c
         break;
       }
       MCM_HIP(hipGraphLaunch(g->exec, P0->stream));
-      for (size_t i = 0; i < n; ++i) G->plans[i]->stepsDone += K;
+      for (size_t i = 0; i < n; ++i) {
+        G->plans[i]->stepsDone += K;
+        G->plans[i]->sensorsRecorded = G->plans[i]->stepsDone;
+      }
     }
     // later work on any plan's stream follows the replays
     MCM_HIP(hipEventRecord(G->events[0], P0->stream));
