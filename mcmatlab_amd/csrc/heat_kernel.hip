@@ -476,15 +476,19 @@ __device__ inline void wave_tridiag(const double (&a)[S], const double (&b)[S], 
 // voxels x = l*S .. l*S+S-1, so every load of the wavefront is contiguous.
 // The right-hand side is the full 7-point explicit stencil (x half-weighted)
 // plus the source, in the expression order of the sequential kernel.
-template <int S>
+template <int S, bool LDS_COEF>
 __global__ void __launch_bounds__(kHeatTPB) heat_sweep_x_wave(HeatArgs<double> a) {
-  __shared__ double sc[kLdsCoef];
+  // the coefficient table in LDS when it fits (LDS_COEF), else read from
+  // global memory; one or the other, so every lookup is a plain LDS or
+  // global load rather than a generic one
+  __shared__ double sc[LDS_COEF ? kLdsCoef : 1];
   step_sensors(a);
-  const int nc = a.nM * a.nM * 3;
-  if (nc <= kLdsCoef)
+  if (LDS_COEF) {
+    const int nc = a.nM * a.nM * 3;
     for (int k = threadIdx.x; k < nc; k += blockDim.x) sc[k] = a.c[k];
-  __syncthreads();
-  const double* __restrict__ cT = nc <= kLdsCoef ? sc : a.c;
+    __syncthreads();
+  }
+  const double* __restrict__ cT = LDS_COEF ? sc : a.c;
   const int lane = __lane_id();
   const int line = blockIdx.x * (kHeatTPB / 64) + (threadIdx.x >> 6);
   const int nx = a.nx, ny = a.ny, nz = a.nz, nxy = nx * ny, nM = a.nM;
@@ -745,13 +749,14 @@ inline int wave_rows(int n) { return n <= 64 ? 1 : n <= 128 ? 2 : n <= 256 ? 4 :
 // come from LDS; the z neighbours are carried in registers.
 constexpr int kTileX = 32, kTileY = 8, kZRun = 16;
 
-template <typename RT>
+template <typename RT, bool LDS_COEF>
 __global__ void __launch_bounds__(kTileX * kTileY) heat_explicit_tile(HeatArgs<RT> a) {
   __shared__ HeatLds<RT> L;
   __shared__ RT sT[kTileY + 2][kTileX + 2];
   __shared__ uint8_t sM[kTileY + 2][kTileX + 2];
   step_sensors(a);
-  const RT* cT = stage_tables(a, L);
+  (void)stage_tables(a, L);
+  const RT* cT = LDS_COEF ? L.c : a.c;  // LDS or global, never a generic pointer
   const int nx = a.nx, ny = a.ny, nz = a.nz, nxy = nx * ny, nM = a.nM;
   const int ntx = (nx + kTileX - 1) / kTileX, nty = (ny + kTileY - 1) / kTileY;
   const int b = blockIdx.x;
@@ -1312,11 +1317,17 @@ void launch_sweep_x(mcm_heat_plan* P, const HeatArgs<RT>& a, unsigned gx) {
   if constexpr (std::is_same<RT, double>::value) {
     const int S = wave_rows(P->nx);
     const unsigned gw = (unsigned)(((int64_t)P->ny * P->nz + kHeatTPB / 64 - 1) / (kHeatTPB / 64));
+    const bool lc = P->nM * P->nM * 3 <= kLdsCoef;
+    auto go = [&](auto sTag) {
+      constexpr int SS = decltype(sTag)::value;
+      if (lc) hipLaunchKernelGGL((heat_sweep_x_wave<SS, true>), dim3(gw), dim3(kHeatTPB), 0, P->stream, a);
+      else hipLaunchKernelGGL((heat_sweep_x_wave<SS, false>), dim3(gw), dim3(kHeatTPB), 0, P->stream, a);
+    };
     switch (S) {
-      case 1: hipLaunchKernelGGL(heat_sweep_x_wave<1>, dim3(gw), dim3(kHeatTPB), 0, P->stream, a); return;
-      case 2: hipLaunchKernelGGL(heat_sweep_x_wave<2>, dim3(gw), dim3(kHeatTPB), 0, P->stream, a); return;
-      case 4: hipLaunchKernelGGL(heat_sweep_x_wave<4>, dim3(gw), dim3(kHeatTPB), 0, P->stream, a); return;
-      case 8: hipLaunchKernelGGL(heat_sweep_x_wave<8>, dim3(gw), dim3(kHeatTPB), 0, P->stream, a); return;
+      case 1: go(std::integral_constant<int, 1>()); return;
+      case 2: go(std::integral_constant<int, 2>()); return;
+      case 4: go(std::integral_constant<int, 4>()); return;
+      case 8: go(std::integral_constant<int, 8>()); return;
       default: break;
     }
   }
@@ -1371,7 +1382,8 @@ void enqueue_step(mcm_heat_plan* P, double dt, int lightsOn, const long long* co
   }
   if (P->explicitScheme) {
     const unsigned ge = (unsigned)(((P->nx + kTileX - 1) / kTileX) * ((P->ny + kTileY - 1) / kTileY) * ((P->nz + kZRun - 1) / kZRun));
-    hipLaunchKernelGGL(heat_explicit_tile<RT>, dim3(ge), dim3(kTileX * kTileY), 0, P->stream, a);
+    if (P->nM * P->nM * 3 <= kLdsCoef) hipLaunchKernelGGL((heat_explicit_tile<RT, true>), dim3(ge), dim3(kTileX * kTileY), 0, P->stream, a);
+    else hipLaunchKernelGGL((heat_explicit_tile<RT, false>), dim3(ge), dim3(kTileX * kTileY), 0, P->stream, a);
     (void)tpb;
   } else {
     launch_sweep_x<RT>(P, a, (unsigned)((P->ny * P->nz + tpb - 1) / tpb));
