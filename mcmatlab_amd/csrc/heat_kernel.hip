@@ -313,11 +313,12 @@ __device__ void sweep_line(const HeatArgs<RT>& a, const RT* cT, const HeatLds<RT
 constexpr int kSweepU = 8;
 
 // substeps 1a + 1b (finiteElementHeatPropagator.c:288-312, :352-389): lane per (y, z) x-line
-template <typename RT>
+template <typename RT, bool LDS_COEF>
 __global__ void __launch_bounds__(kHeatTPB) heat_sweep_x(HeatArgs<RT> a) {
   __shared__ HeatLds<RT> L;
   step_sensors(a);
-  const RT* cT = stage_tables(a, L);
+  (void)stage_tables(a, L);
+  const RT* cT = LDS_COEF ? L.c : a.c;  // LDS or global, never a generic pointer
   const int line = blockIdx.x * blockDim.x + threadIdx.x;
   if (line >= a.ny * a.nz) return;
   const int y = line % a.ny, z = line / a.ny;
@@ -327,10 +328,11 @@ __global__ void __launch_bounds__(kHeatTPB) heat_sweep_x(HeatArgs<RT> a) {
 }
 
 // substeps 2a + 2b (:429-448, :486-521): lane per (x, z) y-line, lanes along x
-template <typename RT>
+template <typename RT, bool LDS_COEF>
 __global__ void __launch_bounds__(kHeatTPB) heat_sweep_y(HeatArgs<RT> a) {
   __shared__ HeatLds<RT> L;
-  const RT* cT = stage_tables(a, L);
+  (void)stage_tables(a, L);
+  const RT* cT = LDS_COEF ? L.c : a.c;  // LDS or global, never a generic pointer
   const int line = blockIdx.x * blockDim.x + threadIdx.x;
   if (line >= a.nx * a.nz) return;
   const int x = line % a.nx, z = line / a.nx;
@@ -340,10 +342,11 @@ __global__ void __launch_bounds__(kHeatTPB) heat_sweep_y(HeatArgs<RT> a) {
 
 // substeps 3a + 3b (:546-565, :606-646) with the Arrhenius update and the
 // per-medium maximum: lane per (x, y) z-line, lanes along x
-template <typename RT>
+template <typename RT, bool LDS_COEF>
 __global__ void __launch_bounds__(kHeatTPB) heat_sweep_z(HeatArgs<RT> a) {
   __shared__ HeatLds<RT> L;
-  const RT* cT = stage_tables(a, L);
+  (void)stage_tables(a, L);
+  const RT* cT = LDS_COEF ? L.c : a.c;  // LDS or global, never a generic pointer
   const int line = blockIdx.x * blockDim.x + threadIdx.x;
   if (line < a.nx * a.ny) {
     const int x = line % a.nx, y = line / a.nx;
@@ -1309,6 +1312,20 @@ int record_sensors(mcm_heat_plan* P, int64_t col) {
   return MCM_OK;
 }
 
+// Lane-per-line y (AXIS 1) or z (AXIS 2) sweep, coefficients in LDS when
+// they fit.
+template <typename RT, int AXIS>
+void launch_lane_sweep(mcm_heat_plan* P, const HeatArgs<RT>& a, unsigned g) {
+  const bool lc = P->nM * P->nM * 3 <= kLdsCoef;
+  if (AXIS == 1) {
+    if (lc) hipLaunchKernelGGL((heat_sweep_y<RT, true>), dim3(g), dim3(kHeatTPB), 0, P->stream, a);
+    else hipLaunchKernelGGL((heat_sweep_y<RT, false>), dim3(g), dim3(kHeatTPB), 0, P->stream, a);
+  } else {
+    if (lc) hipLaunchKernelGGL((heat_sweep_z<RT, true>), dim3(g), dim3(kHeatTPB), 0, P->stream, a);
+    else hipLaunchKernelGGL((heat_sweep_z<RT, false>), dim3(g), dim3(kHeatTPB), 0, P->stream, a);
+  }
+}
+
 // x-sweep launch: the wavefront solver for FP64 lines of up to 512 voxels,
 // else the sequential lane-per-line kernel (FP32 keeps the reference's
 // sequential recurrence).
@@ -1331,7 +1348,8 @@ void launch_sweep_x(mcm_heat_plan* P, const HeatArgs<RT>& a, unsigned gx) {
       default: break;
     }
   }
-  hipLaunchKernelGGL(heat_sweep_x<RT>, dim3(gx), dim3(kHeatTPB), 0, P->stream, a);
+  if (P->nM * P->nM * 3 <= kLdsCoef) hipLaunchKernelGGL((heat_sweep_x<RT, true>), dim3(gx), dim3(kHeatTPB), 0, P->stream, a);
+  else hipLaunchKernelGGL((heat_sweep_x<RT, false>), dim3(gx), dim3(kHeatTPB), 0, P->stream, a);
 }
 
 // y and z sweeps: the wavefront solver for FP64 lines of up to 512 voxels,
@@ -1357,13 +1375,13 @@ template <typename RT>
 void launch_sweep_yz(mcm_heat_plan* P, const HeatArgs<RT>& a, unsigned gy, unsigned gz) {
   if constexpr (std::is_same<RT, double>::value) {
     if (wave_rows(P->ny)) launch_yz_wave<SWEEP_Y>(P, a, P->ny);
-    else hipLaunchKernelGGL(heat_sweep_y<RT>, dim3(gy), dim3(kHeatTPB), 0, P->stream, a);
+    else launch_lane_sweep<RT, 1>(P, a, gy);
     if (wave_rows(P->nz)) launch_yz_wave<SWEEP_Z>(P, a, P->nz);
-    else hipLaunchKernelGGL(heat_sweep_z<RT>, dim3(gz), dim3(kHeatTPB), 0, P->stream, a);
+    else launch_lane_sweep<RT, 2>(P, a, gz);
     return;
   }
-  hipLaunchKernelGGL(heat_sweep_y<RT>, dim3(gy), dim3(kHeatTPB), 0, P->stream, a);
-  hipLaunchKernelGGL(heat_sweep_z<RT>, dim3(gz), dim3(kHeatTPB), 0, P->stream, a);
+  launch_lane_sweep<RT, 1>(P, a, gy);
+  launch_lane_sweep<RT, 2>(P, a, gz);
 }
 
 // The kernels of one time step on the plan's stream; swaps the state buffers.
