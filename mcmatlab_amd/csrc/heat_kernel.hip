@@ -598,10 +598,18 @@ struct YzTile {
   static constexpr int LD = NP + 1;           // LDS pitch of a line (doubles)
 };
 
-template <int S, int KIND>
+// Coefficient tables of up to 8 media (3 * 8 * 8 entries) go to LDS in the
+// tile kernels, whose LDS is mostly taken by the lines.
+constexpr int kTileCoef = 192;
+
+template <int S, int KIND, bool LDS_COEF>
 __global__ void __launch_bounds__(kHeatTPB) heat_sweep_yz_tile(HeatArgs<double> a) {
   constexpr int TX = YzTile<S>::TX, NP = YzTile<S>::NP, LD = YzTile<S>::LD;
   __shared__ double sT[TX * LD], sR[TX * LD];
+  __shared__ double sC[LDS_COEF ? kTileCoef : 1];
+  if (LDS_COEF)
+    for (int k = threadIdx.x; k < a.nM * a.nM * 3; k += blockDim.x) sC[k] = a.c[k];
+  const double* __restrict__ cT = LDS_COEF ? sC : a.c;  // LDS or global, never a generic pointer
   __shared__ uint8_t sM[TX * NP];
   __shared__ double sLogA[KIND == SWEEP_Z ? 256 : 1], sE[KIND == SWEEP_Z ? 256 : 1];
   __shared__ unsigned long long sMax[KIND == SWEEP_Z ? 256 : 1];
@@ -682,16 +690,16 @@ __global__ void __launch_bounds__(kHeatTPB) heat_sweep_yz_tile(HeatArgs<double> 
       double t = R0[s];
       if (!fz) {
         double dTdt = 0;
-        if (k != 0) dTdt -= (TL - Tn[s]) * cc(a, m, ML, d) / 2;
-        if (k != n - 1) dTdt -= (TR - Tn[s]) * cc(a, m, MR, d) / 2;
+        if (k != 0) dTdt -= (TL - Tn[s]) * coef(cT, nM, m, ML, d) / 2;
+        if (k != n - 1) dTdt -= (TR - Tn[s]) * coef(cT, nM, m, MR, d) / 2;
         t = t + dt * dTdt;
         if (k < n - 1) {
-          const double q = dt / 2 * cc(a, m, MR, d);
+          const double q = dt / 2 * coef(cT, nM, m, MR, d);
           cb[s] += q;
           cc_[s] = -q;
         }
         if (k > 0) {
-          const double q = dt / 2 * cc(a, m, ML, d);
+          const double q = dt / 2 * coef(cT, nM, m, ML, d);
           cb[s] += q;
           ca[s] = -q;
         }
@@ -1358,7 +1366,8 @@ template <int S, int KIND>
 void launch_yz_tile(mcm_heat_plan* P, const HeatArgs<double>& a) {
   constexpr int TX = YzTile<S>::TX;
   const unsigned g = (unsigned)(((P->nx + TX - 1) / TX) * (KIND == SWEEP_Y ? P->nz : P->ny));
-  hipLaunchKernelGGL((heat_sweep_yz_tile<S, KIND>), dim3(g), dim3(kHeatTPB), 0, P->stream, a);
+  if (P->nM * P->nM * 3 <= kTileCoef) hipLaunchKernelGGL((heat_sweep_yz_tile<S, KIND, true>), dim3(g), dim3(kHeatTPB), 0, P->stream, a);
+  else hipLaunchKernelGGL((heat_sweep_yz_tile<S, KIND, false>), dim3(g), dim3(kHeatTPB), 0, P->stream, a);
 }
 
 template <int KIND>
