@@ -995,8 +995,8 @@ __global__ void __launch_bounds__(kHeatTPB) heat_adj_solve(HeatArgs<double> a, c
 // (c_(k-1), b_k, a_(k+1)) of A's rows; the neighbouring lane's edge
 // coefficients come by one shuffle each.
 template <int S>
-__device__ inline void adj_line_solve(const HeatArgs<double>& a, int axis, int n, bool lf, const double (&rv)[S], const int (&Mv)[S],
-                                      double (&xs)[S]) {
+__device__ inline void adj_line_solve(const HeatArgs<double>& a, const double* cT, int axis, int n, bool lf, const double (&rv)[S],
+                                      const int (&Mv)[S], double (&xs)[S]) {
   const int lane = __lane_id();
   const double h = a.dt / 2;
   double A[S], B[S], Cc[S];
@@ -1012,12 +1012,12 @@ __device__ inline void adj_line_solve(const HeatArgs<double>& a, int axis, int n
     const int m = Mv[s];
     const int ML = s > 0 ? Mv[s - 1] : Ml, MR = s < S - 1 ? Mv[s + 1] : Mr;
     if (k > 0) {
-      const double q = h * cc(a, m, ML, axis);
+      const double q = h * coef(cT, a.nM, m, ML, axis);
       B[s] += q;
       A[s] = -q;
     }
     if (k < n - 1) {
-      const double q = h * cc(a, m, MR, axis);
+      const double q = h * coef(cT, a.nM, m, MR, axis);
       B[s] += q;
       Cc[s] = -q;
     }
@@ -1038,8 +1038,14 @@ __device__ inline void adj_line_solve(const HeatArgs<double>& a, int axis, int n
 
 // A_d^T y = r with one wavefront per line (lines of up to 512 voxels),
 // loading the line directly (x lines; strided along y and z).
-template <int S, int AXIS>
+template <int S, int AXIS, bool LDS_COEF>
 __global__ void __launch_bounds__(kHeatTPB) heat_adj_solve_wave(HeatArgs<double> a, const double* __restrict__ r, double* y) {
+  __shared__ double sC[LDS_COEF ? kTileCoef : 1];
+  if (LDS_COEF) {
+    for (int k = threadIdx.x; k < a.nM * a.nM * 3; k += blockDim.x) sC[k] = a.c[k];
+    __syncthreads();
+  }
+  const double* __restrict__ cT = LDS_COEF ? sC : a.c;
   const int nx = a.nx, ny = a.ny, nz = a.nz, nxy = nx * ny;
   const int lane = __lane_id();
   const int line = blockIdx.x * (kHeatTPB / 64) + (threadIdx.x >> 6);
@@ -1071,7 +1077,7 @@ __global__ void __launch_bounds__(kHeatTPB) heat_adj_solve_wave(HeatArgs<double>
     rv[s] = k < n ? r[i] : 0.0;
     Mv[s] = a.M[i];
   }
-  adj_line_solve<S>(a, AXIS, n, lf, rv, Mv, xs);
+  adj_line_solve<S>(a, cT, AXIS, n, lf, rv, Mv, xs);
 #pragma unroll
   for (int s = 0; s < S; ++s) {
     const int k = lane * S + s;
@@ -1083,11 +1089,15 @@ __global__ void __launch_bounds__(kHeatTPB) heat_adj_solve_wave(HeatArgs<double>
 // LDS like the forward y/z sweeps: a workgroup takes TX consecutive x-lines
 // at one z (AXIS 1) or y (AXIS 2), loads them row by row (coalesced), each
 // wavefront solves TX/4 lines, and the rows are written back.
-template <int S, int AXIS>
+template <int S, int AXIS, bool LDS_COEF>
 __global__ void __launch_bounds__(kHeatTPB) heat_adj_solve_tile(HeatArgs<double> a, const double* __restrict__ r, double* y) {
   constexpr int TX = YzTile<S>::TX, NP = YzTile<S>::NP, LD = YzTile<S>::LD;
   __shared__ double sR[TX * LD];
   __shared__ uint8_t sM[TX * NP];
+  __shared__ double sC[LDS_COEF ? kTileCoef : 1];
+  if (LDS_COEF)
+    for (int k = threadIdx.x; k < a.nM * a.nM * 3; k += blockDim.x) sC[k] = a.c[k];
+  const double* __restrict__ cT = LDS_COEF ? sC : a.c;
   const int nx = a.nx, ny = a.ny, nz = a.nz, nxy = nx * ny;
   const int n = AXIS == 1 ? ny : nz;
   const int st = AXIS == 1 ? nx : nxy;
@@ -1131,7 +1141,7 @@ __global__ void __launch_bounds__(kHeatTPB) heat_adj_solve_tile(HeatArgs<double>
       rv[s] = k < n ? sR[c * LD + k] : 0.0;
       Mv[s] = k < n ? (int)sM[c * NP + k] : 0;
     }
-    adj_line_solve<S>(a, AXIS, n, lf, rv, Mv, xs);
+    adj_line_solve<S>(a, cT, AXIS, n, lf, rv, Mv, xs);
 #pragma unroll
     for (int s = 0; s < S; ++s) {
       const int k = lane * S + s;
@@ -1150,10 +1160,17 @@ __global__ void __launch_bounds__(kHeatTPB) heat_adj_solve_tile(HeatArgs<double>
 __device__ inline bool heat_frozen(const HeatArgs<double>& a, int x, int y, int z) { return frozen(a, x, y, z); }
 
 // lam_u of the step (plus the damage adjoint of u) and the source adjoint.
+template <bool LDS_COEF>
 __global__ void __launch_bounds__(kHeatTPB) heat_adj_combine(HeatArgs<double> a, const double* __restrict__ mu1, const double* __restrict__ mu2,
                                                             const double* __restrict__ mu3, const double* __restrict__ x0,
                                                             const double* __restrict__ x1, const uint8_t* __restrict__ target,
                                                             double* lamOut, double* dQ, int lightsOn) {
+  __shared__ double sC[LDS_COEF ? kTileCoef : 1];
+  if (LDS_COEF) {
+    for (int k = threadIdx.x; k < a.nM * a.nM * 3; k += blockDim.x) sC[k] = a.c[k];
+    __syncthreads();
+  }
+  const double* __restrict__ cT = LDS_COEF ? sC : a.c;
   const int nx = a.nx, ny = a.ny, nz = a.nz, nxy = nx * ny;
   const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= (int64_t)nxy * nz) return;
@@ -1170,7 +1187,7 @@ __global__ void __launch_bounds__(kHeatTPB) heat_adj_combine(HeatArgs<double> a,
       if (q < 0 || q >= ext[d]) continue;
       const int xx = d == 0 ? q : x, yy = d == 1 ? q : y, zz = d == 2 ? q : z;
       const int mn = a.M[xx + yy * nx + (int64_t)zz * nxy];
-      const double cin = cc(a, mn, m, d), cout = cc(a, m, mn, d);
+      const double cin = coef(cT, a.nM, mn, m, d), cout = coef(cT, a.nM, m, mn, d);
       t1[d] += cin * wv(mu1, xx, yy, zz) - cout * wv(mu1, x, y, z);
       if (d == 1) t2 += cin * wv(mu2, xx, yy, zz) - cout * wv(mu2, x, y, z);
       if (d == 2) t3 += cin * wv(mu3, xx, yy, zz) - cout * wv(mu3, x, y, z);
@@ -1581,13 +1598,15 @@ void adj_solve(mcm_heat_plan* P, const HeatArgs<double>& a, const double* r, dou
   const int n = AXIS == 0 ? P->nx : AXIS == 1 ? P->ny : P->nz;
   const int64_t lines = AXIS == 0 ? (int64_t)P->ny * P->nz : AXIS == 1 ? (int64_t)P->nx * P->nz : (int64_t)P->nx * P->ny;
   const unsigned gw = (unsigned)((lines + kHeatTPB / 64 - 1) / (kHeatTPB / 64));
+  const bool lc = P->nM * P->nM * 3 <= kTileCoef;
   if (AXIS != 0 && wave_rows(n)) {
     // y and z lines through LDS tiles
     auto tile = [&](auto sTag) {
       constexpr int S = decltype(sTag)::value;
       constexpr int TX = YzTile<S>::TX;
       const unsigned g = (unsigned)(((P->nx + TX - 1) / TX) * (AXIS == 1 ? P->nz : P->ny));
-      hipLaunchKernelGGL((heat_adj_solve_tile<S, AXIS == 1 ? 1 : 2>), dim3(g), dim3(kHeatTPB), 0, P->stream, a, r, y);
+      if (lc) hipLaunchKernelGGL((heat_adj_solve_tile<S, AXIS == 1 ? 1 : 2, true>), dim3(g), dim3(kHeatTPB), 0, P->stream, a, r, y);
+      else hipLaunchKernelGGL((heat_adj_solve_tile<S, AXIS == 1 ? 1 : 2, false>), dim3(g), dim3(kHeatTPB), 0, P->stream, a, r, y);
     };
     switch (wave_rows(n)) {
       case 1: tile(std::integral_constant<int, 1>()); return;
@@ -1596,11 +1615,16 @@ void adj_solve(mcm_heat_plan* P, const HeatArgs<double>& a, const double* r, dou
       default: tile(std::integral_constant<int, 8>()); return;
     }
   }
+  auto wave = [&](auto sTag) {
+    constexpr int S = decltype(sTag)::value;
+    if (lc) hipLaunchKernelGGL((heat_adj_solve_wave<S, AXIS, true>), dim3(gw), dim3(kHeatTPB), 0, P->stream, a, r, y);
+    else hipLaunchKernelGGL((heat_adj_solve_wave<S, AXIS, false>), dim3(gw), dim3(kHeatTPB), 0, P->stream, a, r, y);
+  };
   switch (wave_rows(n)) {
-    case 1: hipLaunchKernelGGL((heat_adj_solve_wave<1, AXIS>), dim3(gw), dim3(kHeatTPB), 0, P->stream, a, r, y); return;
-    case 2: hipLaunchKernelGGL((heat_adj_solve_wave<2, AXIS>), dim3(gw), dim3(kHeatTPB), 0, P->stream, a, r, y); return;
-    case 4: hipLaunchKernelGGL((heat_adj_solve_wave<4, AXIS>), dim3(gw), dim3(kHeatTPB), 0, P->stream, a, r, y); return;
-    case 8: hipLaunchKernelGGL((heat_adj_solve_wave<8, AXIS>), dim3(gw), dim3(kHeatTPB), 0, P->stream, a, r, y); return;
+    case 1: wave(std::integral_constant<int, 1>()); return;
+    case 2: wave(std::integral_constant<int, 2>()); return;
+    case 4: wave(std::integral_constant<int, 4>()); return;
+    case 8: wave(std::integral_constant<int, 8>()); return;
     default: hipLaunchKernelGGL(heat_adj_solve<AXIS>, dim3(gLane), dim3(kHeatTPB), 0, P->stream, a, r, y, beta); return;
   }
 }
@@ -1945,8 +1969,12 @@ int mcm_heat_adjoint_run(const mcm_heat_input* in, const mcm_heat_segment* segme
       adj_solve<2>(P, a, (const double*)lam.p, (double*)mu3.p, B, gz);
       adj_solve<1>(P, a, (const double*)mu3.p, (double*)mu2.p, B, gy);
       adj_solve<0>(P, a, (const double*)mu2.p, (double*)mu1.p, B, gx);
-      hipLaunchKernelGGL(heat_adj_combine, dim3(gv), dim3(tpb), 0, s, a, (const double*)mu1.p, (const double*)mu2.p, (const double*)mu3.p, x0,
-                         x1, (const uint8_t*)dMask.p, (double*)lamN.p, (double*)dQ.p, lights[(size_t)n]);
+      if (P->nM * P->nM * 3 <= kTileCoef)
+        hipLaunchKernelGGL(heat_adj_combine<true>, dim3(gv), dim3(tpb), 0, s, a, (const double*)mu1.p, (const double*)mu2.p,
+                           (const double*)mu3.p, x0, x1, (const uint8_t*)dMask.p, (double*)lamN.p, (double*)dQ.p, lights[(size_t)n]);
+      else
+        hipLaunchKernelGGL(heat_adj_combine<false>, dim3(gv), dim3(tpb), 0, s, a, (const double*)mu1.p, (const double*)mu2.p,
+                           (const double*)mu3.p, x0, x1, (const uint8_t*)dMask.p, (double*)lamN.p, (double*)dQ.p, lights[(size_t)n]);
       MCM_HIP(hipGetLastError());
       std::swap(lam.p, lamN.p);
     }
