@@ -506,31 +506,59 @@ __global__ void __launch_bounds__(kHeatTPB) heat_sweep_x_wave(HeatArgs<double> a
   const double dt = a.dt;
   double T[S];
   int Mv[S];
-#pragma unroll
-  for (int s = 0; s < S; ++s) {
-    const int k = lane * S + s;
-    T[s] = k < nx ? T1[i0 + k] : 0.0;
-    Mv[s] = k < nx ? (int)Mg[i0 + k] : 0;
-  }
   // neighbouring lines and the source, loaded up front (clamped addresses)
   double Tym[S], Typ[S], Tzm[S], Tzp[S];
   int Mym[S], Myp[S], Mzm[S], Mzp[S];
   float Qv[S];
+  const int oym = y != 0 ? -nx : 0, oyp = y != ny - 1 ? nx : 0, ozm = z != 0 ? -nxy : 0, ozp = z != nz - 1 ? nxy : 0;
+  const bool vec = S % 4 == 0 && (nx & 3) == 0 && lane * S + S <= nx;
+  if (vec) {
+    // a lane's S voxels of each line are consecutive and 32-byte aligned:
+    // 16-byte loads for T and Q, 4-byte loads for M
+    const int b = i0 + lane * S;
 #pragma unroll
-  for (int s = 0; s < S; ++s) {
-    const int k = lane * S + s;
-    const int i = i0 + (k < nx ? k : 0);
-    const int iym = y != 0 ? i - nx : i, iyp = y != ny - 1 ? i + nx : i;
-    const int izm = z != 0 ? i - nxy : i, izp = z != nz - 1 ? i + nxy : i;
-    Tym[s] = T1[iym];
-    Typ[s] = T1[iyp];
-    Tzm[s] = T1[izm];
-    Tzp[s] = T1[izp];
-    Mym[s] = Mg[iym];
-    Myp[s] = Mg[iyp];
-    Mzm[s] = Mg[izm];
-    Mzp[s] = Mg[izp];
-    Qv[s] = Q[i];
+    for (int s = 0; s < S; s += 2) {
+      const double2 t = *(const double2*)(T1 + b + s), tym = *(const double2*)(T1 + b + oym + s), typ = *(const double2*)(T1 + b + oyp + s);
+      const double2 tzm = *(const double2*)(T1 + b + ozm + s), tzp = *(const double2*)(T1 + b + ozp + s);
+      T[s] = t.x, T[s + 1] = t.y;
+      Tym[s] = tym.x, Tym[s + 1] = tym.y;
+      Typ[s] = typ.x, Typ[s + 1] = typ.y;
+      Tzm[s] = tzm.x, Tzm[s + 1] = tzm.y;
+      Tzp[s] = tzp.x, Tzp[s + 1] = tzp.y;
+    }
+#pragma unroll
+    for (int s = 0; s < S; s += 4) {
+      const float4 q = *(const float4*)(Q + b + s);
+      Qv[s] = q.x, Qv[s + 1] = q.y, Qv[s + 2] = q.z, Qv[s + 3] = q.w;
+      const uint32_t m = *(const uint32_t*)(Mg + b + s), mym = *(const uint32_t*)(Mg + b + oym + s);
+      const uint32_t myp = *(const uint32_t*)(Mg + b + oyp + s), mzm = *(const uint32_t*)(Mg + b + ozm + s);
+      const uint32_t mzp = *(const uint32_t*)(Mg + b + ozp + s);
+#pragma unroll
+      for (int u = 0; u < 4; ++u) {
+        Mv[s + u] = (int)((m >> (8 * u)) & 0xffu);
+        Mym[s + u] = (int)((mym >> (8 * u)) & 0xffu);
+        Myp[s + u] = (int)((myp >> (8 * u)) & 0xffu);
+        Mzm[s + u] = (int)((mzm >> (8 * u)) & 0xffu);
+        Mzp[s + u] = (int)((mzp >> (8 * u)) & 0xffu);
+      }
+    }
+  } else {
+#pragma unroll
+    for (int s = 0; s < S; ++s) {
+      const int k = lane * S + s;
+      const int i = i0 + (k < nx ? k : 0);
+      T[s] = k < nx ? T1[i] : 0.0;
+      Mv[s] = k < nx ? (int)Mg[i] : 0;
+      Tym[s] = T1[i + oym];
+      Typ[s] = T1[i + oyp];
+      Tzm[s] = T1[i + ozm];
+      Tzp[s] = T1[i + ozp];
+      Mym[s] = Mg[i + oym];
+      Myp[s] = Mg[i + oyp];
+      Mzm[s] = Mg[i + ozm];
+      Mzp[s] = Mg[i + ozp];
+      Qv[s] = Q[i];
+    }
   }
   const double Tl = lane_shfl(T[S - 1], lane - 1), Tr = lane_shfl(T[0], lane + 1);
   const int Ml = __shfl(Mv[S - 1], lane > 0 ? lane - 1 : 0, 64), Mr = __shfl(Mv[0], lane < 63 ? lane + 1 : 63, 64);
@@ -572,10 +600,15 @@ __global__ void __launch_bounds__(kHeatTPB) heat_sweep_x_wave(HeatArgs<double> a
   }
   wave_tridiag<S>(ca, cb, cc_, cd, xs);
   double* __restrict__ T2 = a.T2;
+  if (vec) {
 #pragma unroll
-  for (int s = 0; s < S; ++s) {
-    const int k = lane * S + s;
-    if (k < nx) T2[i0 + k] = xs[s];
+    for (int s = 0; s < S; s += 2) *(double2*)(T2 + i0 + lane * S + s) = make_double2(xs[s], xs[s + 1]);
+  } else {
+#pragma unroll
+    for (int s = 0; s < S; ++s) {
+      const int k = lane * S + s;
+      if (k < nx) T2[i0 + k] = xs[s];
+    }
   }
 }
 
