@@ -359,19 +359,20 @@ __global__ void __launch_bounds__(kHeatTPB) heat_sweep_z(HeatArgs<RT> a) {
 }
 
 // ---------------------------------------------------------------------------
-// Wavefront-parallel tridiagonal solve (FP64 x-sweep)
+// Wavefront-parallel tridiagonal solve (FP64 sweeps)
 //
-// A system  a[r] x[r-1] + b[r] x[r] + c[r] x[r+1] = d[r]  of up to 64*S rows
-// is spread over the 64 lanes of a wavefront, S consecutive rows per lane
-// (row r = lane * S + s); unused rows are a = c = 0, b = 1, d = 0.  The
-// matrices of the heat step are strictly diagonally dominant, so no pivoting
-// is needed.  Partition method:
+// A system  a[r] x[r-1] + b[r] x[r] + c[r] x[r+1] = d[r]  of up to W*S rows
+// is spread over a segment of W lanes (W = 64, 32 or 16; a wavefront solves
+// 64 / W systems side by side), S consecutive rows per lane (row
+// r = lane * S + s, lane counted within the segment); unused rows are
+// a = c = 0, b = 1, d = 0.  The matrices of the heat step are strictly
+// diagonally dominant, so no pivoting is needed.  Partition method:
 //   1. each lane eliminates its interior rows 1..S-2, expressing them through
 //      its own first and last unknowns;
 //   2. the first unknown of each lane is eliminated as well, leaving one
 //      tridiagonal equation per lane in the lanes' last unknowns U;
-//   3. that 64-row system is solved by parallel cyclic reduction, six
-//      steps of lane shuffles;
+//   3. that W-row system is solved by parallel cyclic reduction, log2(W)
+//      steps of lane shuffles that never cross a segment's edge;
 //   4. each lane recovers its first and interior unknowns from U.
 // Same solution as the sequential Thomas recurrence up to rounding.
 __device__ inline double lane_shfl(double v, int src) { return __shfl(v, src < 0 ? 0 : (src > 63 ? 63 : src), 64); }
@@ -386,10 +387,12 @@ __device__ inline double pdiv(double a, double b) {
   return a * r;
 }
 
-template <int S>
+template <int S, int W = 64>
 __device__ inline void wave_tridiag(const double (&a)[S], const double (&b)[S], const double (&c)[S], const double (&d)[S],
                                     double (&x)[S]) {
-  const int lane = __lane_id();
+  // a line per W-lane segment of the wavefront; lane is the index within it
+  static_assert(W == 16 || W == 32 || W == 64, "segments of 16, 32 or 64 lanes");
+  const int wl = __lane_id(), lane = wl & (W - 1);
   double al[S], be[S], ga[S], de[S];
   double A1 = a[0], B1 = b[0], C1 = c[0], D1 = d[0];
   double A2 = a[S - 1], B2 = b[S - 1], C2 = c[S - 1], D2 = d[S - 1];
@@ -434,9 +437,9 @@ __device__ inline void wave_tridiag(const double (&a)[S], const double (&b)[S], 
   } else {
     // shuffles run in every lane (a lane reading from a lane that is
     // inactive at the shuffle gets 0), then the edge lanes are masked
-    const bool hasNext = lane < 63;
-    const double sA = lane_shfl(A1, lane + 1), sB = lane_shfl(B1, lane + 1);
-    const double sC = lane_shfl(C1, lane + 1), sD = lane_shfl(D1, lane + 1);
+    const bool hasNext = lane < W - 1;
+    const double sA = lane_shfl(A1, wl + 1), sB = lane_shfl(B1, wl + 1);
+    const double sC = lane_shfl(C1, wl + 1), sD = lane_shfl(D1, wl + 1);
     const double nA = hasNext ? sA : 0.0, nB = hasNext ? sB : 1.0, nC = hasNext ? sC : 0.0, nD = hasNext ? sD : 0.0;
     const double iB1 = pdiv(1.0, B1), inB = pdiv(1.0, nB);
     pa = -A2 * A1 * iB1;
@@ -445,12 +448,12 @@ __device__ inline void wave_tridiag(const double (&a)[S], const double (&b)[S], 
     pd = D2 - A2 * D1 * iB1 - C2 * nD * inB;
   }
 #pragma unroll
-  for (int h = 1; h < 64; h <<= 1) {
-    const bool lo = lane >= h, hi = lane + h < 64;
-    const double sam = lane_shfl(pa, lane - h), sbm = lane_shfl(pb, lane - h);
-    const double scm = lane_shfl(pc, lane - h), sdm = lane_shfl(pd, lane - h);
-    const double sap = lane_shfl(pa, lane + h), sbp = lane_shfl(pb, lane + h);
-    const double scp = lane_shfl(pc, lane + h), sdp = lane_shfl(pd, lane + h);
+  for (int h = 1; h < W; h <<= 1) {
+    const bool lo = lane >= h, hi = lane + h < W;
+    const double sam = lane_shfl(pa, wl - h), sbm = lane_shfl(pb, wl - h);
+    const double scm = lane_shfl(pc, wl - h), sdm = lane_shfl(pd, wl - h);
+    const double sap = lane_shfl(pa, wl + h), sbp = lane_shfl(pb, wl + h);
+    const double scp = lane_shfl(pc, wl + h), sdp = lane_shfl(pd, wl + h);
     const double am = lo ? sam : 0.0, bm = lo ? sbm : 1.0, cm = lo ? scm : 0.0, dm = lo ? sdm : 0.0;
     const double ap = hi ? sap : 0.0, bp = hi ? sbp : 1.0, cp = hi ? scp : 0.0, dp = hi ? sdp : 0.0;
     const double k1 = pdiv(pa, bm), k2 = pdiv(pc, bp);
@@ -463,7 +466,7 @@ __device__ inline void wave_tridiag(const double (&a)[S], const double (&b)[S], 
   if constexpr (S == 1) {
     x[0] = U;
   } else {
-    const double sU = lane_shfl(U, lane - 1);
+    const double sU = lane_shfl(U, wl - 1);
     const double Up = lane > 0 ? sU : 0.0;
     const double first = pdiv(D1 - A1 * Up - C1 * U, B1);
     x[0] = first;
@@ -479,7 +482,7 @@ __device__ inline void wave_tridiag(const double (&a)[S], const double (&b)[S], 
 // voxels x = l*S .. l*S+S-1, so every load of the wavefront is contiguous.
 // The right-hand side is the full 7-point explicit stencil (x half-weighted)
 // plus the source, in the expression order of the sequential kernel.
-template <int S, bool LDS_COEF>
+template <int S, int W, bool LDS_COEF>
 __global__ void __launch_bounds__(kHeatTPB) heat_sweep_x_wave(HeatArgs<double> a) {
   // the coefficient table in LDS when it fits (LDS_COEF), else read from
   // global memory; one or the other, so every lookup is a plain LDS or
@@ -492,11 +495,13 @@ __global__ void __launch_bounds__(kHeatTPB) heat_sweep_x_wave(HeatArgs<double> a
     __syncthreads();
   }
   const double* __restrict__ cT = LDS_COEF ? sc : a.c;
-  const int lane = __lane_id();
-  const int line = blockIdx.x * (kHeatTPB / 64) + (threadIdx.x >> 6);
+  // a line per W-lane segment; a segment past the last line solves the
+  // last line again and stores nothing
+  const int wl = __lane_id(), lane = wl & (W - 1);
+  const int line = (blockIdx.x * (kHeatTPB / 64) + (threadIdx.x >> 6)) * (64 / W) + wl / W;
   const int nx = a.nx, ny = a.ny, nz = a.nz, nxy = nx * ny, nM = a.nM;
-  if (line >= ny * nz) return;
-  const int y = line % ny, z = line / ny;
+  const bool valid = line < ny * nz;
+  const int y = (valid ? line : ny * nz - 1) % ny, z = (valid ? line : ny * nz - 1) / ny;
   const int i0 = y * nx + z * nxy;
   const bool sk = a.sinked;
   const bool lf = sk && (y == 0 || y == ny - 1 || z == 0 || z == nz - 1);
@@ -560,8 +565,8 @@ __global__ void __launch_bounds__(kHeatTPB) heat_sweep_x_wave(HeatArgs<double> a
       Qv[s] = Q[i];
     }
   }
-  const double Tl = lane_shfl(T[S - 1], lane - 1), Tr = lane_shfl(T[0], lane + 1);
-  const int Ml = __shfl(Mv[S - 1], lane > 0 ? lane - 1 : 0, 64), Mr = __shfl(Mv[0], lane < 63 ? lane + 1 : 63, 64);
+  const double Tl = lane_shfl(T[S - 1], wl - 1), Tr = lane_shfl(T[0], wl + 1);
+  const int Ml = __shfl(Mv[S - 1], wl > 0 ? wl - 1 : 0, 64), Mr = __shfl(Mv[0], wl < 63 ? wl + 1 : 63, 64);
   double ca[S], cb[S], cc_[S], cd[S], xs[S];
 #pragma unroll
   for (int s = 0; s < S; ++s) {
@@ -598,8 +603,9 @@ __global__ void __launch_bounds__(kHeatTPB) heat_sweep_x_wave(HeatArgs<double> a
       ca[s] = -q;
     }
   }
-  wave_tridiag<S>(ca, cb, cc_, cd, xs);
+  wave_tridiag<S, W>(ca, cb, cc_, cd, xs);
   double* __restrict__ T2 = a.T2;
+  if (!valid) return;
   if (vec) {
 #pragma unroll
     for (int s = 0; s < S; s += 2) *(double2*)(T2 + i0 + lane * S + s) = make_double2(xs[s], xs[s + 1]);
@@ -621,23 +627,23 @@ __global__ void __launch_bounds__(kHeatTPB) heat_sweep_x_wave(HeatArgs<double> a
 // l*S .. l*S+S-1), and the workgroup writes the rows back.  The right-hand
 // side is T1 of the previous substep minus dt/2 L_d T(step start) along the
 // line, in the expression order of the sequential kernel.
-template <int S>
+template <int S, int W>
 struct YzTile {
+  static constexpr int NP = W * S;            // padded line length
+  static constexpr int LD = NP + 1;           // LDS pitch of a line (doubles)
   // lines per workgroup: 16 (128-byte rows) up to 128 voxels a line; longer
   // lines would need ~70 KB of LDS per workgroup at 16, leaving only two
   // workgroups per CU, so they take 8
-  static constexpr int TX = S <= 2 ? 16 : 8;
-  static constexpr int NP = 64 * S;           // padded line length
-  static constexpr int LD = NP + 1;           // LDS pitch of a line (doubles)
+  static constexpr int TX = NP <= 128 ? 16 : 8;
 };
 
 // Coefficient tables of up to 8 media (3 * 8 * 8 entries) go to LDS in the
 // tile kernels, whose LDS is mostly taken by the lines.
 constexpr int kTileCoef = 192;
 
-template <int S, int KIND, bool LDS_COEF>
+template <int S, int W, int KIND, bool LDS_COEF>
 __global__ void __launch_bounds__(kHeatTPB) heat_sweep_yz_tile(HeatArgs<double> a) {
-  constexpr int TX = YzTile<S>::TX, NP = YzTile<S>::NP, LD = YzTile<S>::LD;
+  constexpr int TX = YzTile<S, W>::TX, NP = YzTile<S, W>::NP, LD = YzTile<S, W>::LD;
   __shared__ double sT[TX * LD], sR[TX * LD];
   __shared__ double sC[LDS_COEF ? kTileCoef : 1];
   if (LDS_COEF)
@@ -647,7 +653,7 @@ __global__ void __launch_bounds__(kHeatTPB) heat_sweep_yz_tile(HeatArgs<double> 
   __shared__ double sLogA[KIND == SWEEP_Z ? 256 : 1], sE[KIND == SWEEP_Z ? 256 : 1];
   __shared__ unsigned long long sMax[KIND == SWEEP_Z ? 256 : 1];
   const int nx = a.nx, ny = a.ny, nz = a.nz, nxy = nx * ny, nM = a.nM;
-  const int n = KIND == SWEEP_Y ? ny : nz;
+  const int nLine = KIND == SWEEP_Y ? ny : nz;
   const int st = KIND == SWEEP_Y ? nx : nxy;
   const int d = KIND == SWEEP_Y ? 1 : 2;
   const int nOther = KIND == SWEEP_Y ? nz : ny;
@@ -672,7 +678,7 @@ __global__ void __launch_bounds__(kHeatTPB) heat_sweep_yz_tile(HeatArgs<double> 
     for (int q = 0; q < kIt; ++q) {
       const int e = threadIdx.x + q * kHeatTPB;
       const int c = e % TX, k = e / TX;
-      const bool ok = c < cols && k < n;
+      const bool ok = c < cols && k < nLine;
       const int i = ok ? base + c + k * st : base;
       vT[q] = T1[i];
       vR[q] = a.T2[i];
@@ -682,7 +688,7 @@ __global__ void __launch_bounds__(kHeatTPB) heat_sweep_yz_tile(HeatArgs<double> 
     for (int q = 0; q < kIt; ++q) {
       const int e = threadIdx.x + q * kHeatTPB;
       const int c = e % TX, k = e / TX;
-      if (c < cols && k < n) {
+      if (c < cols && k < nLine) {
         sT[c * LD + k] = vT[q];
         sR[c * LD + k] = vR[q];
         sM[c * NP + k] = (uint8_t)vM[q];
@@ -690,10 +696,16 @@ __global__ void __launch_bounds__(kHeatTPB) heat_sweep_yz_tile(HeatArgs<double> 
     }
   }
   __syncthreads();
-  const int lane = __lane_id(), wave = threadIdx.x >> 6;
+  // a line per W-lane segment: segment g of wavefront w solves lines
+  // w * 64 / W + g, + 4 * 64 / W, ...; a segment past the last line solves
+  // an empty system and stores nothing
+  const int wl = __lane_id(), lane = wl & (W - 1), wave = threadIdx.x >> 6;
   const bool sk = a.sinked;
   const double dt = a.dt;
-  for (int c = wave; c < cols; c += kHeatTPB / 64) {
+  for (int c0 = wave * (64 / W); c0 < cols; c0 += kHeatTPB / W) {
+    const bool valid = c0 + wl / W < cols;
+    const int c = valid ? c0 + wl / W : c0;
+    const int n = valid ? nLine : 0;
     const int x = tx0 + c;
     const bool lf = sk && (x == 0 || x == nx - 1 || o == 0 || o == nOther - 1);
     double Tn[S], R0[S];
@@ -705,8 +717,8 @@ __global__ void __launch_bounds__(kHeatTPB) heat_sweep_yz_tile(HeatArgs<double> 
       R0[s] = k < n ? sR[c * LD + k] : 0.0;
       Mv[s] = k < n ? (int)sM[c * NP + k] : 0;
     }
-    const double Tl = lane_shfl(Tn[S - 1], lane - 1), Tr = lane_shfl(Tn[0], lane + 1);
-    const int Ml = __shfl(Mv[S - 1], lane > 0 ? lane - 1 : 0, 64), Mr = __shfl(Mv[0], lane < 63 ? lane + 1 : 63, 64);
+    const double Tl = lane_shfl(Tn[S - 1], wl - 1), Tr = lane_shfl(Tn[0], wl + 1);
+    const int Ml = __shfl(Mv[S - 1], wl > 0 ? wl - 1 : 0, 64), Mr = __shfl(Mv[0], wl < 63 ? wl + 1 : 63, 64);
     double ca[S], cb[S], cc_[S], cd[S], xs[S];
 #pragma unroll
     for (int s = 0; s < S; ++s) {
@@ -739,7 +751,7 @@ __global__ void __launch_bounds__(kHeatTPB) heat_sweep_yz_tile(HeatArgs<double> 
       }
       cd[s] = t;
     }
-    wave_tridiag<S>(ca, cb, cc_, cd, xs);
+    wave_tridiag<S, W>(ca, cb, cc_, cd, xs);
 #pragma unroll
     for (int s = 0; s < S; ++s) {
       const int k = lane * S + s;
@@ -756,7 +768,7 @@ __global__ void __launch_bounds__(kHeatTPB) heat_sweep_yz_tile(HeatArgs<double> 
     for (int q = 0; q < kIt; ++q) {
       const int e = threadIdx.x + q * kHeatTPB;
       const int c = e % TX, k = e / TX;
-      const bool ok = c < cols && k < n;
+      const bool ok = c < cols && k < nLine;
       vO[q] = Om[ok ? base + c + k * st : base];
     }
   }
@@ -764,7 +776,7 @@ __global__ void __launch_bounds__(kHeatTPB) heat_sweep_yz_tile(HeatArgs<double> 
   for (int q = 0; q < kIt; ++q) {
     const int e = threadIdx.x + q * kHeatTPB;
     const int c = e % TX, k = e / TX;
-    if (c >= cols || k >= n) continue;
+    if (c >= cols || k >= nLine) continue;
     const int i = base + c + k * st;
     const double t = sR[c * LD + k];
     T2[i] = t;
@@ -783,6 +795,49 @@ __global__ void __launch_bounds__(kHeatTPB) heat_sweep_yz_tile(HeatArgs<double> 
 // rows per lane of the wavefront x-solve for a line of n voxels (0: use the
 // sequential lane-per-line kernel)
 inline int wave_rows(int n) { return n <= 64 ? 1 : n <= 128 ? 2 : n <= 256 ? 4 : n <= 512 ? 8 : 0; }
+
+// rows per lane S and lanes per line W of the forward wavefront solver for a
+// line of n <= 512 voxels.  Fewer lanes per line mean more rows per lane and
+// fewer cyclic-reduction steps, with 64 / W lines solved side by side in one
+// wavefront.  Measured per step (200 on + 200 off steps, layered skin):
+// y/z sweeps with W = 16 up to 128 voxels and W = 32 up to 256 took the DG-ADI
+// step at 128^3 from 0.097 to 0.064 ms and at 200^3 from 0.259 to 0.232 ms;
+// the x sweep, which also gathers four neighbouring lines, gains from W = 32
+// only up to 128 voxels (more rows per lane cost it occupancy: 154 VGPRs at
+// S = 8).
+struct WaveShape {
+  int S, W;
+};
+inline WaveShape wave_shape(int n, bool xAxis) {
+  if (n <= 32 || n > 256) return {wave_rows(n), 64};
+  if (xAxis) return n <= 128 ? WaveShape{n <= 64 ? 2 : 4, 32} : WaveShape{4, 64};
+  return n <= 128 ? WaveShape{n <= 64 ? 4 : 8, 16} : WaveShape{8, 32};
+}
+
+// calls f(integral_constant S, integral_constant W) for the shapes
+// wave_shape picks (X: the x sweep's, else the y/z sweeps')
+template <bool X, class F>
+inline void with_wave_shape(WaveShape ws, F&& f) {
+  using I1 = std::integral_constant<int, 1>;
+  using I2 = std::integral_constant<int, 2>;
+  using I4 = std::integral_constant<int, 4>;
+  using I8 = std::integral_constant<int, 8>;
+  using I16 = std::integral_constant<int, 16>;
+  using I32 = std::integral_constant<int, 32>;
+  using I64 = std::integral_constant<int, 64>;
+  const int key = ws.W * 16 + ws.S;
+  if (key == 64 * 16 + 1) return f(I1(), I64());
+  if (key == 64 * 16 + 8) return f(I8(), I64());
+  if constexpr (X) {
+    if (key == 64 * 16 + 4) return f(I4(), I64());
+    if (key == 32 * 16 + 2) return f(I2(), I32());
+    if (key == 32 * 16 + 4) return f(I4(), I32());
+  } else {
+    if (key == 32 * 16 + 8) return f(I8(), I32());
+    if (key == 16 * 16 + 4) return f(I4(), I16());
+    if (key == 16 * 16 + 8) return f(I8(), I16());
+  }
+}
 
 // Forward Euler step (no reference counterpart; the 7-point right-hand side
 // of substep 1a with all three axes fully weighted, the damage update and
@@ -1124,7 +1179,7 @@ __global__ void __launch_bounds__(kHeatTPB) heat_adj_solve_wave(HeatArgs<double>
 // wavefront solves TX/4 lines, and the rows are written back.
 template <int S, int AXIS, bool LDS_COEF>
 __global__ void __launch_bounds__(kHeatTPB) heat_adj_solve_tile(HeatArgs<double> a, const double* __restrict__ r, double* y) {
-  constexpr int TX = YzTile<S>::TX, NP = YzTile<S>::NP, LD = YzTile<S>::LD;
+  constexpr int TX = YzTile<S, 64>::TX, NP = YzTile<S, 64>::NP, LD = YzTile<S, 64>::LD;
   __shared__ double sR[TX * LD];
   __shared__ uint8_t sM[TX * NP];
   __shared__ double sC[LDS_COEF ? kTileCoef : 1];
@@ -1390,20 +1445,16 @@ void launch_lane_sweep(mcm_heat_plan* P, const HeatArgs<RT>& a, unsigned g) {
 template <typename RT>
 void launch_sweep_x(mcm_heat_plan* P, const HeatArgs<RT>& a, unsigned gx) {
   if constexpr (std::is_same<RT, double>::value) {
-    const int S = wave_rows(P->nx);
-    const unsigned gw = (unsigned)(((int64_t)P->ny * P->nz + kHeatTPB / 64 - 1) / (kHeatTPB / 64));
-    const bool lc = P->nM * P->nM * 3 <= kLdsCoef;
-    auto go = [&](auto sTag) {
-      constexpr int SS = decltype(sTag)::value;
-      if (lc) hipLaunchKernelGGL((heat_sweep_x_wave<SS, true>), dim3(gw), dim3(kHeatTPB), 0, P->stream, a);
-      else hipLaunchKernelGGL((heat_sweep_x_wave<SS, false>), dim3(gw), dim3(kHeatTPB), 0, P->stream, a);
-    };
-    switch (S) {
-      case 1: go(std::integral_constant<int, 1>()); return;
-      case 2: go(std::integral_constant<int, 2>()); return;
-      case 4: go(std::integral_constant<int, 4>()); return;
-      case 8: go(std::integral_constant<int, 8>()); return;
-      default: break;
+    if (wave_rows(P->nx)) {
+      const bool lc = P->nM * P->nM * 3 <= kLdsCoef;
+      with_wave_shape<true>(wave_shape(P->nx, true), [&](auto sTag, auto wTag) {
+        constexpr int SS = decltype(sTag)::value, WW = decltype(wTag)::value;
+        const int64_t lines = (int64_t)P->ny * P->nz, perBlock = kHeatTPB / WW;
+        const unsigned gw = (unsigned)((lines + perBlock - 1) / perBlock);
+        if (lc) hipLaunchKernelGGL((heat_sweep_x_wave<SS, WW, true>), dim3(gw), dim3(kHeatTPB), 0, P->stream, a);
+        else hipLaunchKernelGGL((heat_sweep_x_wave<SS, WW, false>), dim3(gw), dim3(kHeatTPB), 0, P->stream, a);
+      });
+      return;
     }
   }
   if (P->nM * P->nM * 3 <= kLdsCoef) hipLaunchKernelGGL((heat_sweep_x<RT, true>), dim3(gx), dim3(kHeatTPB), 0, P->stream, a);
@@ -1412,22 +1463,19 @@ void launch_sweep_x(mcm_heat_plan* P, const HeatArgs<RT>& a, unsigned gx) {
 
 // y and z sweeps: the wavefront solver for FP64 lines of up to 512 voxels,
 // else the sequential lane-per-line kernels.
-template <int S, int KIND>
+template <int S, int W, int KIND>
 void launch_yz_tile(mcm_heat_plan* P, const HeatArgs<double>& a) {
-  constexpr int TX = YzTile<S>::TX;
+  constexpr int TX = YzTile<S, W>::TX;
   const unsigned g = (unsigned)(((P->nx + TX - 1) / TX) * (KIND == SWEEP_Y ? P->nz : P->ny));
-  if (P->nM * P->nM * 3 <= kTileCoef) hipLaunchKernelGGL((heat_sweep_yz_tile<S, KIND, true>), dim3(g), dim3(kHeatTPB), 0, P->stream, a);
-  else hipLaunchKernelGGL((heat_sweep_yz_tile<S, KIND, false>), dim3(g), dim3(kHeatTPB), 0, P->stream, a);
+  if (P->nM * P->nM * 3 <= kTileCoef) hipLaunchKernelGGL((heat_sweep_yz_tile<S, W, KIND, true>), dim3(g), dim3(kHeatTPB), 0, P->stream, a);
+  else hipLaunchKernelGGL((heat_sweep_yz_tile<S, W, KIND, false>), dim3(g), dim3(kHeatTPB), 0, P->stream, a);
 }
 
 template <int KIND>
 void launch_yz_wave(mcm_heat_plan* P, const HeatArgs<double>& a, int n) {
-  switch (wave_rows(n)) {
-    case 1: launch_yz_tile<1, KIND>(P, a); return;
-    case 2: launch_yz_tile<2, KIND>(P, a); return;
-    case 4: launch_yz_tile<4, KIND>(P, a); return;
-    default: launch_yz_tile<8, KIND>(P, a); return;
-  }
+  with_wave_shape<false>(wave_shape(n, false), [&](auto sTag, auto wTag) {
+    launch_yz_tile<decltype(sTag)::value, decltype(wTag)::value, KIND>(P, a);
+  });
 }
 
 template <typename RT>
@@ -1636,7 +1684,7 @@ void adj_solve(mcm_heat_plan* P, const HeatArgs<double>& a, const double* r, dou
     // y and z lines through LDS tiles
     auto tile = [&](auto sTag) {
       constexpr int S = decltype(sTag)::value;
-      constexpr int TX = YzTile<S>::TX;
+      constexpr int TX = YzTile<S, 64>::TX;
       const unsigned g = (unsigned)(((P->nx + TX - 1) / TX) * (AXIS == 1 ? P->nz : P->ny));
       if (lc) hipLaunchKernelGGL((heat_adj_solve_tile<S, AXIS == 1 ? 1 : 2, true>), dim3(g), dim3(kHeatTPB), 0, P->stream, a, r, y);
       else hipLaunchKernelGGL((heat_adj_solve_tile<S, AXIS == 1 ? 1 : 2, false>), dim3(g), dim3(kHeatTPB), 0, P->stream, a, r, y);

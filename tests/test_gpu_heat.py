@@ -145,12 +145,15 @@ def line_case(nx, ny, nz, bt, seed=0, steps=6):
 
 @pytest.mark.parametrize("bt", [0, 1])
 @pytest.mark.parametrize("axis", [0, 1, 2])
-@pytest.mark.parametrize("n", [1, 64, 65, 129, 300, 512, 513])
+@pytest.mark.parametrize("n", [1, 32, 33, 64, 65, 128, 129, 200, 256, 257, 300, 512, 513])
 def test_gpu_heat_line_lengths(api, n, axis, bt):
-    """FP64 lines of up to 512 voxels are solved by one wavefront each (rows
-    per lane 1, 2, 4, 8), longer ones by the lane-per-line recurrence."""
+    """FP64 lines of up to 512 voxels are solved by the wavefront solver, a
+    line per segment of 64, 32 or 16 lanes with 1, 2, 4 or 8 rows per lane
+    (the lengths cover every shape and both ends of its range), longer ones
+    by the lane-per-line recurrence.  The other two extents are odd, so
+    some wavefronts hold a segment with no line of its own."""
     print(f"[gpu-heat] axis {axis} line {n} bt {bt}", flush=True)
-    shape = [3, 4, 5]
+    shape = [3, 5, 7]
     shape[axis] = n
     T0, Om0, P = line_case(*shape, bt, seed=n + 1000 * axis)
     api.finiteElementHeatPropagator(T0, Om0, P, precision="fp64")
