@@ -1082,13 +1082,13 @@ __global__ void __launch_bounds__(kHeatTPB) heat_adj_solve(HeatArgs<double> a, c
 // the whole line is frozen).  The rows of A^T are (sub, diag, super) =
 // (c_(k-1), b_k, a_(k+1)) of A's rows; the neighbouring lane's edge
 // coefficients come by one shuffle each.
-template <int S>
+template <int S, int W>
 __device__ inline void adj_line_solve(const HeatArgs<double>& a, const double* cT, int axis, int n, bool lf, const double (&rv)[S],
                                       const int (&Mv)[S], double (&xs)[S]) {
-  const int lane = __lane_id();
+  const int wl = __lane_id(), lane = wl & (W - 1);  // lane within the line's W-lane segment
   const double h = a.dt / 2;
   double A[S], B[S], Cc[S];
-  const int Ml = __shfl(Mv[S - 1], lane > 0 ? lane - 1 : 0, 64), Mr = __shfl(Mv[0], lane < 63 ? lane + 1 : 63, 64);
+  const int Ml = __shfl(Mv[S - 1], wl > 0 ? wl - 1 : 0, 64), Mr = __shfl(Mv[0], wl < 63 ? wl + 1 : 63, 64);
   // rows of A
 #pragma unroll
   for (int s = 0; s < S; ++s) {
@@ -1111,7 +1111,7 @@ __device__ inline void adj_line_solve(const HeatArgs<double>& a, const double* c
     }
   }
   // rows of A^T
-  const double cPrevLane = lane_shfl(Cc[S - 1], lane - 1), aNextLane = lane_shfl(A[0], lane + 1);
+  const double cPrevLane = lane_shfl(Cc[S - 1], wl - 1), aNextLane = lane_shfl(A[0], wl + 1);
   double ta[S], tb[S], tc[S];
 #pragma unroll
   for (int s = 0; s < S; ++s) {
@@ -1121,12 +1121,12 @@ __device__ inline void adj_line_solve(const HeatArgs<double>& a, const double* c
     tb[s] = ok ? B[s] : 1.0;
     tc[s] = ok && k < n - 1 ? (s < S - 1 ? A[s + 1] : aNextLane) : 0.0;
   }
-  wave_tridiag<S>(ta, tb, tc, rv, xs);
+  wave_tridiag<S, W>(ta, tb, tc, rv, xs);
 }
 
 // A_d^T y = r with one wavefront per line (lines of up to 512 voxels),
 // loading the line directly (x lines; strided along y and z).
-template <int S, int AXIS, bool LDS_COEF>
+template <int S, int W, int AXIS, bool LDS_COEF>
 __global__ void __launch_bounds__(kHeatTPB) heat_adj_solve_wave(HeatArgs<double> a, const double* __restrict__ r, double* y) {
   __shared__ double sC[LDS_COEF ? kTileCoef : 1];
   if (LDS_COEF) {
@@ -1135,27 +1135,28 @@ __global__ void __launch_bounds__(kHeatTPB) heat_adj_solve_wave(HeatArgs<double>
   }
   const double* __restrict__ cT = LDS_COEF ? sC : a.c;
   const int nx = a.nx, ny = a.ny, nz = a.nz, nxy = nx * ny;
-  const int lane = __lane_id();
-  const int line = blockIdx.x * (kHeatTPB / 64) + (threadIdx.x >> 6);
-  int i0, st, n, nLines;
+  // a line per W-lane segment; a segment past the last line solves the
+  // last line again and stores nothing
+  const int wl = __lane_id(), lane = wl & (W - 1);
+  const int seg = (blockIdx.x * (kHeatTPB / 64) + (threadIdx.x >> 6)) * (64 / W) + wl / W;
+  const int nLines = AXIS == 0 ? ny * nz : AXIS == 1 ? nx * nz : nxy;
+  const bool valid = seg < nLines;
+  const int line = valid ? seg : nLines - 1;
+  int i0, st, n;
   bool lf;
   if (AXIS == 0) {
-    nLines = ny * nz;
     const int yy = line % ny, zz = line / ny;
     i0 = yy * nx + zz * nxy, st = 1, n = nx;
     lf = a.sinked && (yy == 0 || yy == ny - 1 || zz == 0 || zz == nz - 1);
   } else if (AXIS == 1) {
-    nLines = nx * nz;
     const int xx = line % nx, zz = line / nx;
     i0 = xx + zz * nxy, st = nx, n = ny;
     lf = a.sinked && (xx == 0 || xx == nx - 1 || zz == 0 || zz == nz - 1);
   } else {
-    nLines = nxy;
     i0 = line, st = nxy, n = nz;
     const int xx = line % nx, yy = line / nx;
     lf = a.sinked && (xx == 0 || xx == nx - 1 || yy == 0 || yy == ny - 1);
   }
-  if (line >= nLines) return;
   double rv[S], xs[S];
   int Mv[S];
 #pragma unroll
@@ -1165,7 +1166,8 @@ __global__ void __launch_bounds__(kHeatTPB) heat_adj_solve_wave(HeatArgs<double>
     rv[s] = k < n ? r[i] : 0.0;
     Mv[s] = a.M[i];
   }
-  adj_line_solve<S>(a, cT, AXIS, n, lf, rv, Mv, xs);
+  adj_line_solve<S, W>(a, cT, AXIS, n, lf, rv, Mv, xs);
+  if (!valid) return;
 #pragma unroll
   for (int s = 0; s < S; ++s) {
     const int k = lane * S + s;
@@ -1177,9 +1179,9 @@ __global__ void __launch_bounds__(kHeatTPB) heat_adj_solve_wave(HeatArgs<double>
 // LDS like the forward y/z sweeps: a workgroup takes TX consecutive x-lines
 // at one z (AXIS 1) or y (AXIS 2), loads them row by row (coalesced), each
 // wavefront solves TX/4 lines, and the rows are written back.
-template <int S, int AXIS, bool LDS_COEF>
+template <int S, int W, int AXIS, bool LDS_COEF>
 __global__ void __launch_bounds__(kHeatTPB) heat_adj_solve_tile(HeatArgs<double> a, const double* __restrict__ r, double* y) {
-  constexpr int TX = YzTile<S, 64>::TX, NP = YzTile<S, 64>::NP, LD = YzTile<S, 64>::LD;
+  constexpr int TX = YzTile<S, W>::TX, NP = YzTile<S, W>::NP, LD = YzTile<S, W>::LD;
   __shared__ double sR[TX * LD];
   __shared__ uint8_t sM[TX * NP];
   __shared__ double sC[LDS_COEF ? kTileCoef : 1];
@@ -1187,7 +1189,7 @@ __global__ void __launch_bounds__(kHeatTPB) heat_adj_solve_tile(HeatArgs<double>
     for (int k = threadIdx.x; k < a.nM * a.nM * 3; k += blockDim.x) sC[k] = a.c[k];
   const double* __restrict__ cT = LDS_COEF ? sC : a.c;
   const int nx = a.nx, ny = a.ny, nz = a.nz, nxy = nx * ny;
-  const int n = AXIS == 1 ? ny : nz;
+  const int nLine = AXIS == 1 ? ny : nz;
   const int st = AXIS == 1 ? nx : nxy;
   const int nOther = AXIS == 1 ? nz : ny;
   const int ntx = (nx + TX - 1) / TX;
@@ -1202,7 +1204,7 @@ __global__ void __launch_bounds__(kHeatTPB) heat_adj_solve_tile(HeatArgs<double>
     for (int q = 0; q < kIt; ++q) {
       const int e = threadIdx.x + q * kHeatTPB;
       const int c = e % TX, k = e / TX;
-      const int i = c < cols && k < n ? base + c + k * st : base;
+      const int i = c < cols && k < nLine ? base + c + k * st : base;
       vR[q] = r[i];
       vM[q] = a.M[i];
     }
@@ -1210,15 +1212,20 @@ __global__ void __launch_bounds__(kHeatTPB) heat_adj_solve_tile(HeatArgs<double>
     for (int q = 0; q < kIt; ++q) {
       const int e = threadIdx.x + q * kHeatTPB;
       const int c = e % TX, k = e / TX;
-      if (c < cols && k < n) {
+      if (c < cols && k < nLine) {
         sR[c * LD + k] = vR[q];
         sM[c * NP + k] = (uint8_t)vM[q];
       }
     }
   }
   __syncthreads();
-  const int lane = __lane_id(), wave = threadIdx.x >> 6;
-  for (int c = wave; c < cols; c += kHeatTPB / 64) {
+  // a line per W-lane segment; a segment past the last line solves an
+  // empty system and stores nothing
+  const int wl = __lane_id(), lane = wl & (W - 1), wave = threadIdx.x >> 6;
+  for (int c0 = wave * (64 / W); c0 < cols; c0 += kHeatTPB / W) {
+    const bool valid = c0 + wl / W < cols;
+    const int c = valid ? c0 + wl / W : c0;
+    const int n = valid ? nLine : 0;
     const int x = tx0 + c;
     const bool lf = a.sinked && (x == 0 || x == nx - 1 || o == 0 || o == nOther - 1);
     double rv[S], xs[S];
@@ -1229,7 +1236,7 @@ __global__ void __launch_bounds__(kHeatTPB) heat_adj_solve_tile(HeatArgs<double>
       rv[s] = k < n ? sR[c * LD + k] : 0.0;
       Mv[s] = k < n ? (int)sM[c * NP + k] : 0;
     }
-    adj_line_solve<S>(a, cT, AXIS, n, lf, rv, Mv, xs);
+    adj_line_solve<S, W>(a, cT, AXIS, n, lf, rv, Mv, xs);
 #pragma unroll
     for (int s = 0; s < S; ++s) {
       const int k = lane * S + s;
@@ -1241,7 +1248,7 @@ __global__ void __launch_bounds__(kHeatTPB) heat_adj_solve_tile(HeatArgs<double>
   for (int q = 0; q < kIt; ++q) {
     const int e = threadIdx.x + q * kHeatTPB;
     const int c = e % TX, k = e / TX;
-    if (c < cols && k < n) y[base + c + k * st] = sR[c * LD + k];
+    if (c < cols && k < nLine) y[base + c + k * st] = sR[c * LD + k];
   }
 }
 
@@ -1678,35 +1685,27 @@ template <int AXIS>
 void adj_solve(mcm_heat_plan* P, const HeatArgs<double>& a, const double* r, double* y, double* beta, unsigned gLane) {
   const int n = AXIS == 0 ? P->nx : AXIS == 1 ? P->ny : P->nz;
   const int64_t lines = AXIS == 0 ? (int64_t)P->ny * P->nz : AXIS == 1 ? (int64_t)P->nx * P->nz : (int64_t)P->nx * P->ny;
-  const unsigned gw = (unsigned)((lines + kHeatTPB / 64 - 1) / (kHeatTPB / 64));
   const bool lc = P->nM * P->nM * 3 <= kTileCoef;
-  if (AXIS != 0 && wave_rows(n)) {
-    // y and z lines through LDS tiles
-    auto tile = [&](auto sTag) {
-      constexpr int S = decltype(sTag)::value;
-      constexpr int TX = YzTile<S, 64>::TX;
-      const unsigned g = (unsigned)(((P->nx + TX - 1) / TX) * (AXIS == 1 ? P->nz : P->ny));
-      if (lc) hipLaunchKernelGGL((heat_adj_solve_tile<S, AXIS == 1 ? 1 : 2, true>), dim3(g), dim3(kHeatTPB), 0, P->stream, a, r, y);
-      else hipLaunchKernelGGL((heat_adj_solve_tile<S, AXIS == 1 ? 1 : 2, false>), dim3(g), dim3(kHeatTPB), 0, P->stream, a, r, y);
-    };
-    switch (wave_rows(n)) {
-      case 1: tile(std::integral_constant<int, 1>()); return;
-      case 2: tile(std::integral_constant<int, 2>()); return;
-      case 4: tile(std::integral_constant<int, 4>()); return;
-      default: tile(std::integral_constant<int, 8>()); return;
-    }
+  if (!wave_rows(n)) {
+    hipLaunchKernelGGL(heat_adj_solve<AXIS>, dim3(gLane), dim3(kHeatTPB), 0, P->stream, a, r, y, beta);
+    return;
   }
-  auto wave = [&](auto sTag) {
-    constexpr int S = decltype(sTag)::value;
-    if (lc) hipLaunchKernelGGL((heat_adj_solve_wave<S, AXIS, true>), dim3(gw), dim3(kHeatTPB), 0, P->stream, a, r, y);
-    else hipLaunchKernelGGL((heat_adj_solve_wave<S, AXIS, false>), dim3(gw), dim3(kHeatTPB), 0, P->stream, a, r, y);
-  };
-  switch (wave_rows(n)) {
-    case 1: wave(std::integral_constant<int, 1>()); return;
-    case 2: wave(std::integral_constant<int, 2>()); return;
-    case 4: wave(std::integral_constant<int, 4>()); return;
-    case 8: wave(std::integral_constant<int, 8>()); return;
-    default: hipLaunchKernelGGL(heat_adj_solve<AXIS>, dim3(gLane), dim3(kHeatTPB), 0, P->stream, a, r, y, beta); return;
+  if constexpr (AXIS != 0) {
+    // y and z lines through LDS tiles, in the shapes of the forward sweeps
+    with_wave_shape<false>(wave_shape(n, false), [&](auto sTag, auto wTag) {
+      constexpr int S = decltype(sTag)::value, W = decltype(wTag)::value;
+      constexpr int TX = YzTile<S, W>::TX;
+      const unsigned g = (unsigned)(((P->nx + TX - 1) / TX) * (AXIS == 1 ? P->nz : P->ny));
+      if (lc) hipLaunchKernelGGL((heat_adj_solve_tile<S, W, AXIS == 1 ? 1 : 2, true>), dim3(g), dim3(kHeatTPB), 0, P->stream, a, r, y);
+      else hipLaunchKernelGGL((heat_adj_solve_tile<S, W, AXIS == 1 ? 1 : 2, false>), dim3(g), dim3(kHeatTPB), 0, P->stream, a, r, y);
+    });
+  } else {
+    with_wave_shape<true>(wave_shape(n, true), [&](auto sTag, auto wTag) {
+      constexpr int S = decltype(sTag)::value, W = decltype(wTag)::value;
+      const unsigned gw = (unsigned)((lines + kHeatTPB / W - 1) / (kHeatTPB / W));
+      if (lc) hipLaunchKernelGGL((heat_adj_solve_wave<S, W, AXIS, true>), dim3(gw), dim3(kHeatTPB), 0, P->stream, a, r, y);
+      else hipLaunchKernelGGL((heat_adj_solve_wave<S, W, AXIS, false>), dim3(gw), dim3(kHeatTPB), 0, P->stream, a, r, y);
+    });
   }
 }
 

@@ -94,10 +94,9 @@ def test_adjoint_dQ_matches_central_differences(api, bt):
     assert abs(rich - 1) <= 1e-4, (dJ1 / ad1, dJ2 / ad2, rich)
 
 
-@pytest.mark.parametrize("shape", [(520, 4, 3), (3, 600, 4), (4, 3, 513), (5, 512, 3)])
-def test_adjoint_long_lines(api, shape):
-    """Lines beyond one wavefront's reach (more than 512 voxels) take the
-    lane-per-line recurrence; 512 is the longest wavefront solve."""
+def fd_check_shape(api, shape):
+    """The adjoint's dJ/dT0 along a random direction against central
+    differences of J on a block of the given shape."""
     T0, Om0, P, segs = case(1, shape=shape, seed=5)
     segs = [(6, 2e-5, True), (3, 3e-5, False)]
     r = api.heat_adjoint(T0, Om0, P, segments=segs, checkpointEvery=4)
@@ -110,6 +109,21 @@ def test_adjoint_long_lines(api, shape):
     fd = (Jp - Jm) / (2 * h)
     ad = float(np.sum(r["dJ_dT0"] * d))
     assert abs(fd - ad) <= 1e-6 * abs(ad) + 1e-12 * r["J"], (fd, ad)
+
+
+@pytest.mark.parametrize("shape", [(520, 4, 3), (3, 600, 4), (4, 3, 513), (5, 512, 3)])
+def test_adjoint_long_lines(api, shape):
+    """Lines beyond one wavefront's reach (more than 512 voxels) take the
+    lane-per-line recurrence; 512 is the longest wavefront solve."""
+    fd_check_shape(api, shape)
+
+
+@pytest.mark.parametrize("shape", [(40, 7, 5), (100, 5, 3), (200, 3, 5), (5, 40, 7), (3, 100, 5), (7, 5, 200), (3, 5, 129)])
+def test_adjoint_segment_shapes(api, shape):
+    """Lines of 33..256 voxels are solved on 16- or 32-lane segments, several
+    lines per wavefront; odd line counts leave some segments without a line."""
+    print(f"[gpu-adjoint] shape {shape}", flush=True)
+    fd_check_shape(api, shape)
 
 
 def test_adjoint_lights_off_has_no_source_gradient(api):
