@@ -376,6 +376,39 @@ __global__ void __launch_bounds__(kHeatTPB) heat_sweep_z(HeatArgs<RT> a) {
 //   4. each lane recovers its first and interior unknowns from U.
 // Same solution as the sequential Thomas recurrence up to rounding.
 __device__ inline double lane_shfl(double v, int src) { return __shfl(v, src < 0 ? 0 : (src > 63 ? 63 : src), 64); }
+__device__ inline int lane_shfl(int v, int src) { return __shfl(v, src < 0 ? 0 : (src > 63 ? 63 : src), 64); }
+
+// DPP move within rows of 16 lanes (CTRL 0x100 + h: row_shl:h, lane p gets
+// lane p + h; 0x110 + h: row_shr:h, lane p gets lane p - h; 0 where the
+// source lies outside the row)
+template <int CTRL>
+__device__ inline int dpp_move(int v) { return __builtin_amdgcn_update_dpp(0, v, CTRL, 0xF, 0xF, true); }
+template <int CTRL>
+__device__ inline double dpp_move(double v) {
+  const long long b = __double_as_longlong(v);
+  const int lo = dpp_move<CTRL>((int)b), hi = dpp_move<CTRL>((int)(b >> 32));
+  return __longlong_as_double(((long long)hi << 32) | (unsigned)lo);
+}
+
+// v of lane wl + h (UP) or wl - h of the same W-lane segment; a lane whose
+// source lies outside its segment gets an unspecified value, which callers
+// mask.  16-lane segments are the DPP unit's rows, so their shifts are VALU
+// row moves rather than LDS-routed permutes.
+template <int W, bool UP, typename V>
+__device__ inline V seg_shift(V v, int h) {
+  if constexpr (W == 16) {
+    constexpr int base = UP ? 0x100 : 0x110;
+    switch (h) {
+      case 1: return dpp_move<base + 1>(v);
+      case 2: return dpp_move<base + 2>(v);
+      case 4: return dpp_move<base + 4>(v);
+      case 8: return dpp_move<base + 8>(v);
+      default: break;
+    }
+  }
+  const int wl = __lane_id();
+  return lane_shfl(v, UP ? wl + h : wl - h);
+}
 
 // a / b for the positive, well-scaled pivots of the solver: hardware
 // reciprocal estimate refined by two Newton steps (a few ulp), instead of the
@@ -438,8 +471,8 @@ __device__ inline void wave_tridiag(const double (&a)[S], const double (&b)[S], 
     // shuffles run in every lane (a lane reading from a lane that is
     // inactive at the shuffle gets 0), then the edge lanes are masked
     const bool hasNext = lane < W - 1;
-    const double sA = lane_shfl(A1, wl + 1), sB = lane_shfl(B1, wl + 1);
-    const double sC = lane_shfl(C1, wl + 1), sD = lane_shfl(D1, wl + 1);
+    const double sA = seg_shift<W, true>(A1, 1), sB = seg_shift<W, true>(B1, 1);
+    const double sC = seg_shift<W, true>(C1, 1), sD = seg_shift<W, true>(D1, 1);
     const double nA = hasNext ? sA : 0.0, nB = hasNext ? sB : 1.0, nC = hasNext ? sC : 0.0, nD = hasNext ? sD : 0.0;
     const double iB1 = pdiv(1.0, B1), inB = pdiv(1.0, nB);
     pa = -A2 * A1 * iB1;
@@ -450,10 +483,10 @@ __device__ inline void wave_tridiag(const double (&a)[S], const double (&b)[S], 
 #pragma unroll
   for (int h = 1; h < W; h <<= 1) {
     const bool lo = lane >= h, hi = lane + h < W;
-    const double sam = lane_shfl(pa, wl - h), sbm = lane_shfl(pb, wl - h);
-    const double scm = lane_shfl(pc, wl - h), sdm = lane_shfl(pd, wl - h);
-    const double sap = lane_shfl(pa, wl + h), sbp = lane_shfl(pb, wl + h);
-    const double scp = lane_shfl(pc, wl + h), sdp = lane_shfl(pd, wl + h);
+    const double sam = seg_shift<W, false>(pa, h), sbm = seg_shift<W, false>(pb, h);
+    const double scm = seg_shift<W, false>(pc, h), sdm = seg_shift<W, false>(pd, h);
+    const double sap = seg_shift<W, true>(pa, h), sbp = seg_shift<W, true>(pb, h);
+    const double scp = seg_shift<W, true>(pc, h), sdp = seg_shift<W, true>(pd, h);
     const double am = lo ? sam : 0.0, bm = lo ? sbm : 1.0, cm = lo ? scm : 0.0, dm = lo ? sdm : 0.0;
     const double ap = hi ? sap : 0.0, bp = hi ? sbp : 1.0, cp = hi ? scp : 0.0, dp = hi ? sdp : 0.0;
     const double k1 = pdiv(pa, bm), k2 = pdiv(pc, bp);
@@ -466,7 +499,7 @@ __device__ inline void wave_tridiag(const double (&a)[S], const double (&b)[S], 
   if constexpr (S == 1) {
     x[0] = U;
   } else {
-    const double sU = lane_shfl(U, wl - 1);
+    const double sU = seg_shift<W, false>(U, 1);
     const double Up = lane > 0 ? sU : 0.0;
     const double first = pdiv(D1 - A1 * Up - C1 * U, B1);
     x[0] = first;
@@ -565,8 +598,8 @@ __global__ void __launch_bounds__(kHeatTPB) heat_sweep_x_wave(HeatArgs<double> a
       Qv[s] = Q[i];
     }
   }
-  const double Tl = lane_shfl(T[S - 1], wl - 1), Tr = lane_shfl(T[0], wl + 1);
-  const int Ml = __shfl(Mv[S - 1], wl > 0 ? wl - 1 : 0, 64), Mr = __shfl(Mv[0], wl < 63 ? wl + 1 : 63, 64);
+  const double Tl = seg_shift<W, false>(T[S - 1], 1), Tr = seg_shift<W, true>(T[0], 1);
+  const int Ml = seg_shift<W, false>(Mv[S - 1], 1), Mr = seg_shift<W, true>(Mv[0], 1);
   double ca[S], cb[S], cc_[S], cd[S], xs[S];
 #pragma unroll
   for (int s = 0; s < S; ++s) {
@@ -717,8 +750,8 @@ __global__ void __launch_bounds__(kHeatTPB) heat_sweep_yz_tile(HeatArgs<double> 
       R0[s] = k < n ? sR[c * LD + k] : 0.0;
       Mv[s] = k < n ? (int)sM[c * NP + k] : 0;
     }
-    const double Tl = lane_shfl(Tn[S - 1], wl - 1), Tr = lane_shfl(Tn[0], wl + 1);
-    const int Ml = __shfl(Mv[S - 1], wl > 0 ? wl - 1 : 0, 64), Mr = __shfl(Mv[0], wl < 63 ? wl + 1 : 63, 64);
+    const double Tl = seg_shift<W, false>(Tn[S - 1], 1), Tr = seg_shift<W, true>(Tn[0], 1);
+    const int Ml = seg_shift<W, false>(Mv[S - 1], 1), Mr = seg_shift<W, true>(Mv[0], 1);
     double ca[S], cb[S], cc_[S], cd[S], xs[S];
 #pragma unroll
     for (int s = 0; s < S; ++s) {
@@ -1088,7 +1121,7 @@ __device__ inline void adj_line_solve(const HeatArgs<double>& a, const double* c
   const int wl = __lane_id(), lane = wl & (W - 1);  // lane within the line's W-lane segment
   const double h = a.dt / 2;
   double A[S], B[S], Cc[S];
-  const int Ml = __shfl(Mv[S - 1], wl > 0 ? wl - 1 : 0, 64), Mr = __shfl(Mv[0], wl < 63 ? wl + 1 : 63, 64);
+  const int Ml = seg_shift<W, false>(Mv[S - 1], 1), Mr = seg_shift<W, true>(Mv[0], 1);
   // rows of A
 #pragma unroll
   for (int s = 0; s < S; ++s) {
@@ -1111,7 +1144,7 @@ __device__ inline void adj_line_solve(const HeatArgs<double>& a, const double* c
     }
   }
   // rows of A^T
-  const double cPrevLane = lane_shfl(Cc[S - 1], wl - 1), aNextLane = lane_shfl(A[0], wl + 1);
+  const double cPrevLane = seg_shift<W, false>(Cc[S - 1], 1), aNextLane = seg_shift<W, true>(A[0], 1);
   double ta[S], tb[S], tc[S];
 #pragma unroll
   for (int s = 0; s < S; ++s) {
