@@ -274,6 +274,41 @@ def bench_adjoint(n=128, steps=3000, every=100):
             "dJ_dQ_max": float(abs(r["dJ_dQ"]).max())}
 
 
+def bench_binning(n=200, bins=150, sample_div=16):
+    """Media binning (getDownsampledParamVals) of a temperature-dependent
+    medium filling the 200^3 grid into 150 sub-media (Example14's nBins):
+    one call, host input to host output (host sums, upload, 151 passes over
+    the 8e6 sets, download).  CPU: the oracle (the reference's sequential
+    algorithm, one thread) on every 16th set with the same number of bins."""
+    import numpy as np
+
+    from mcmatlab_amd import api
+    from mcmatlab_amd.workloads import thermochromic_params
+
+    pv = thermochromic_params(n=n)
+    api.getDownsampledParamVals(pv[:, :4096], 4)  # device initialisation
+    t = time.perf_counter()
+    out, bins_ = api.getDownsampledParamVals(pv, bins)
+    dt = time.perf_counter() - t
+    n1 = pv.shape[1]
+    res = {"metric": "set-passes/s, media binning of 200^3 voxel parameter sets into 150 bins", "value": n1 * bins / dt,
+           "unit": "set-passes/s", "seconds": dt, "n1": n1, "n2": bins, "dtype": "f32",
+           "labels_used": int(len(np.unique(bins_)))}
+    sys.path.insert(0, os.path.join(os.path.dirname(os.path.abspath(__file__)), "tests"))
+    try:
+        import oracle_lib as OL
+
+        sample = np.ascontiguousarray(pv[:, ::sample_div])
+        t = time.perf_counter()
+        OL.oracle_binning(sample, bins)
+        dc = time.perf_counter() - t
+        res["cpu_baseline"] = {"value": sample.shape[1] * bins / dc, "unit": "set-passes/s", "cores": 1, "kind": "port",
+                               "sample": f"every {sample_div}th set ({sample.shape[1]}), {bins} bins ({dc:.1f} s)"}
+    except OSError:
+        res["cpu_baseline"] = None
+    return res
+
+
 def bench_sweep(n_models=32, seed=3, heat_steps=1000, every=100, parts=("mc", "grad", "heat", "adjoint")):
     """Batched forward + backward sweep over independent models on uneven
     grids (multi_grid_sweep: sides 64..247 from seed 3, cyclic x/y, 1e6-1e7
@@ -376,11 +411,12 @@ def main():
     ap.add_argument("--no-heat", action="store_true")
     ap.add_argument("--workload", default="layered_tissue_200_fresnel_tophat",
                     choices=["layered_tissue_200_fresnel_tophat", "spectral64", "heat200", "heat200explicit", "grad128", "adjoint128",
-                             "sweep32"],
+                             "sweep32", "binning200"],
                     help="secondary workloads (each prints its own line): spectral64 batched vs plain launches of the "
                          "64-wavelength workload; heat200 (heat200explicit) the DG-ADI (explicit) heat propagator alone; "
                          "grad128 absorbed-power gradients; "
-                         "adjoint128 the heat adjoint; sweep32 the 32-grid forward + backward sweep")
+                         "adjoint128 the heat adjoint; sweep32 the 32-grid forward + backward sweep; "
+                         "binning200 the media binning of a 200^3 temperature-dependent medium")
     ap.add_argument("--copies", type=int, default=0, choices=[0, 1, 2, 4, 8],
                     help="force the number of XCD-private fluence copies (0: the autotuner's choice)")
     ap.add_argument("--sweep-parts", default="mc,grad,heat,adjoint",
@@ -393,7 +429,7 @@ def main():
         sys.exit(2)
 
     secondary = {"spectral64": bench_spectral, "heat200": bench_heat_line, "heat200explicit": lambda: bench_heat_line("explicit"),
-                 "grad128": bench_grad, "adjoint128": bench_adjoint, "sweep32": bench_sweep}
+                 "grad128": bench_grad, "adjoint128": bench_adjoint, "sweep32": bench_sweep, "binning200": bench_binning}
     if args.workload in secondary:
         kw = {"parts": tuple(args.sweep_parts.split(","))} if args.workload == "sweep32" else {}
         print(json.dumps(secondary[args.workload](**kw)), flush=True)

@@ -336,6 +336,20 @@ typedef struct mcm_heat_adjoint_output {
 int mcm_heat_adjoint_run(const mcm_heat_input *in, const mcm_heat_segment *segments, int64_t nSegments,
                          const uint8_t *targetMask, int64_t checkpointEvery, mcm_heat_adjoint_output *out);
 
+/* ---- Media binning ------------------------------------------------------
+ * [downsampledParamVals, binidx] = getDownsampledParamVals(paramVals, N2)
+ * (replaces the MEX +MCmatlab/src/getDownsampledParamVals.c:54-202, called
+ * from getOpticalMediaProperties.m:157 and getThermalMediaProperties.m:80).
+ * paramVals: host [3 x n1] single, column-major (one (mua, mus, g) or
+ * (VHC, TC, 1) set per voxel of a medium); downsampled: host [3 x n2];
+ * binidx: host [n1], 1-based label of the nearest downsampled set (uint8,
+ * wrapping like the reference's for n2 > 255).  Farthest-point sampling in
+ * the range-normalised space of the rows that vary, with the reference's
+ * arithmetic and its single-threaded selection order, on GPU `device`.
+ * n1, n2 >= 1; non-finite values are MCM_ERR_INVALID_ARGUMENT. */
+int mcm_downsample_params(const float *paramVals, int64_t n1, int64_t n2, float *downsampled, uint8_t *binidx,
+                          int32_t device);
+
 #ifdef __cplusplus
 }
 #endif

@@ -13,6 +13,12 @@ arguments and outputs:
       @model/simulateHeatDistribution.m:259-264) -- `steps` heat steps of
       size dt on the GPU.
 
+  downsampledParamVals, binidx = getDownsampledParamVals(paramVals, N2)
+      (+MCmatlab/src/getDownsampledParamVals.c:54; called by
+      @model/private/getOpticalMediaProperties.m:157 and
+      getThermalMediaProperties.m:80) -- farthest-point binning of a
+      medium's per-voxel parameter sets on the GPU.
+
 Errors raise MCMError (the MEX files raise MATLAB errors via
 mexErrMsgIdAndTxt).  There is no CPU fallback.
 """
@@ -195,6 +201,23 @@ class HeatGroup:
             self.close()
         except Exception:
             pass
+
+
+def getDownsampledParamVals(paramVals, N2, device=0):
+    """[downsampledParamVals, binidx] = getDownsampledParamVals(paramVals, N2)
+    on GPU `device`: paramVals is 3 x N1 (rows mua, mus, g, or VHC, TC, 1),
+    single precision; returns the 3 x N2 single downsampled sets and the
+    1-based uint8 label (length N1) of the set each input set is nearest to,
+    bit for bit the reference's outputs."""
+    pv = np.asarray(paramVals, dtype=np.float32)
+    if pv.ndim != 2 or pv.shape[0] != 3:
+        raise ValueError("paramVals must be 3 x N1")
+    n1, n2 = int(pv.shape[1]), int(N2)
+    flat = np.ascontiguousarray(pv.ravel(order="F"))
+    out = np.zeros(3 * max(n2, 1), np.float32)
+    bins = np.zeros(max(n1, 1), np.uint8)
+    check(lib().mcm_downsample_params(flat.ctypes.data, n1, n2, out.ctypes.data, bins.ctypes.data, int(device)))
+    return out[:3 * n2].reshape(3, n2, order="F"), bins[:n1]
 
 
 def heat_adjoint(T, Omega, P, segments=None, targetMask=None, checkpointEvery=0, grads=True, T_f64=None, Omega_f64=None):

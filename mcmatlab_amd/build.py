@@ -40,6 +40,8 @@ SOURCES = [
     # the heat stencils are compared with the FP64 oracle operation for
     # operation, so they are built without FMA contraction
     ("heat_kernel.hip", ["-ffp-contract=off"]),
+    # the binning distances are compared with the reference bit for bit
+    ("binning_kernel.hip", ["-ffp-contract=off"]),
 ]
 
 

@@ -240,3 +240,19 @@ def _sweep_heat(NFR, M, mua, nM, d, power=1.0, steps=1000, dt=5e-6):
          "tempSensorInterpWeights": np.zeros((0, 3)), "lightsOn": True, "steps": steps, "dt": dt}
     T0 = np.full(M.shape, 37.0, np.float32)
     return T0, np.zeros(M.shape, np.float32), P
+
+
+def thermochromic_params(n=200, seed=4):
+    """Per-voxel optical parameter sets (mua, mus, g) of one temperature-
+    dependent medium filling an n^3 grid, as getOpticalMediaProperties.m
+    collects them for binning (3 x n^3 single, one column per voxel, x
+    fastest): a heated spot (Gaussian, 37 -> 77 degC, with 0.2 degC noise)
+    drives a thermochromic absorber, mua = 0.5 + 9.5 * s and mus = 200 -
+    60 * s with s = (T - 37) / 40, and g = 0.9 throughout (a constant row)."""
+    rng = np.random.default_rng(seed)
+    ax = (np.arange(n) - (n - 1) / 2) / n
+    X, Y, Z = np.meshgrid(ax, ax, (np.arange(n) + 0.5) / n, indexing="ij")
+    T = 37 + 40 * np.exp(-(X ** 2 + Y ** 2) / 0.02 - Z ** 2 / 0.08) + 0.2 * rng.standard_normal(X.shape)
+    s = ((T - 37) / 40).ravel(order="F")
+    return np.vstack([0.5 + 9.5 * s, 200 - 60 * s, np.full(s.size, 0.9)]).astype(np.float32)
+

@@ -16,6 +16,10 @@
  *                    order (pinned bit for bit against the reference build),
  *                    the same scheme in FP64, and an FP64 explicit-Euler
  *                    variant (pinned by closed forms in tests/test_oracle_heat.py).
+ *   oracle_downsample_params follows +MCmatlab/src/getDownsampledParamVals.c
+ *                    (farthest-point media binning), single-threaded build
+ *                    (pinned bit for bit against the reference build and by
+ *                    brute-force cases in tests/test_oracle_binning.py).
  */
 #ifndef MCM_ORACLE_H
 #define MCM_ORACLE_H
@@ -45,6 +49,11 @@ int oracle_mc_run(const mcm_mc_input *in, mcm_mc_output *out, int nthreads, int 
 #define ORC_HEAT_FP64_DGADI 1
 #define ORC_HEAT_FP64_EXPLICIT 2
 int oracle_heat_run(const mcm_heat_input *in, mcm_heat_output *out, int variant, int nthreads);
+
+/* [out, binidx] = getDownsampledParamVals(pv, N2): pv is 3 x N1 (mua, mus, g)
+ * column-major single, out 3 x N2, binidx N1 (1-based).  0, or -1 / -2 on bad
+ * arguments / allocation failure. */
+int oracle_downsample_params(const float *pv, int64_t N1, int64_t N2, float *out, uint8_t *binidx);
 
 /* Philox4x32-10 (Salmon et al., SC'11), exposed for the RNG known-answer test. */
 void oracle_philox4x32_10(const uint32_t ctr[4], const uint32_t key[2], uint32_t out[4]);

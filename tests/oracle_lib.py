@@ -16,6 +16,7 @@ PORT_OMP_LIB = os.path.join(ORACLE_DIR, "_build", "liboracle_omp.so")
 REF_MC_LIB = os.path.join(ORACLE_DIR, "_ref", "libref_mcmatlab.so")
 REF_HEAT_LIB = os.path.join(ORACLE_DIR, "_ref", "libref_heat.so")
 REF_MC_OMP_LIB = os.path.join(ORACLE_DIR, "_ref", "libref_mcmatlab_omp.so")
+REF_BINNING_LIB = os.path.join(ORACLE_DIR, "_ref", "libref_binning.so")
 
 RNG_DSFMT = 0
 RNG_PHILOX = 1
@@ -42,6 +43,7 @@ def load_oracle(omp=False):
         lib.oracle_philox4x32_10.argtypes = [C.POINTER(C.c_uint32), C.POINTER(C.c_uint32), C.POINTER(C.c_uint32)]
         lib.oracle_dsfmt_stream.argtypes = [C.c_uint32, C.c_int, C.POINTER(C.c_double)]
         lib.oracle_last_error.restype = C.c_char_p
+        lib.oracle_downsample_params.argtypes = [C.c_void_p, C.c_int64, C.c_int64, C.c_void_p, C.c_void_p]
         _libs[path] = lib
     return _libs[path]
 
@@ -75,6 +77,48 @@ def oracle_heat(T, Omega, P, variant=HEAT_FP64_DGADI, want_f64=True, T_f64=None,
     if rc != 0:
         raise RuntimeError("oracle_heat_run failed (%d)" % rc)
     return out.result()
+
+
+def oracle_binning(paramVals, n2):
+    """[downsampledParamVals, binidx] = getDownsampledParamVals(paramVals, n2)
+    through the C restatement; paramVals is 3 x N1 single."""
+    pv = np.asfortranarray(np.asarray(paramVals, dtype=np.float32).reshape(3, -1))
+    n1 = pv.shape[1]
+    flat = np.ascontiguousarray(pv.ravel(order="F"))
+    out = np.zeros(3 * n2, np.float32)
+    bins = np.zeros(n1, np.uint8)
+    rc = load_oracle().oracle_downsample_params(flat.ctypes.data, n1, n2, out.ctypes.data, bins.ctypes.data)
+    if rc != 0:
+        raise RuntimeError("oracle_downsample_params failed (%d)" % rc)
+    return out.reshape(3, n2, order="F"), bins
+
+
+def ref_binning(paramVals, n2):
+    """The same through the reference's getDownsampledParamVals.c compiled
+    from source against the mx shim (only where oracle/_ref is built)."""
+    from mexdriver import MexLib, mxSINGLE
+
+    m = _ref_binning_lib()
+    pv = np.asarray(paramVals, dtype=np.float32).reshape(3, -1)
+    a = m.num(pv, mxSINGLE, shape=pv.shape)
+    b = m.num(float(n2))
+    try:
+        outs = m.call(2, [a, b])
+        res = m.array(outs[0]), m.array(outs[1]).reshape(-1)
+        for o in outs:
+            m.lib.mxs_destroy(o)
+        return res
+    finally:
+        m.lib.mxs_destroy(a)
+        m.lib.mxs_destroy(b)
+
+
+def _ref_binning_lib():
+    from mexdriver import MexLib
+
+    if REF_BINNING_LIB not in _libs:
+        _libs[REF_BINNING_LIB] = MexLib(REF_BINNING_LIB)
+    return _libs[REF_BINNING_LIB]
 
 
 def philox(ctr, key):
