@@ -102,7 +102,7 @@ def build_test_harnesses(verbose=True):
             print("  built tests/_build/libmc_host.so")
     mexdir = os.path.join(PKG, "mex")
     shim = os.path.join(ROOT, "oracle", "mexshim")
-    for name in ("MCmatlab_mex.c", "finiteElementHeatPropagator_mex.c"):
+    for name in ("MCmatlab_mex.c", "finiteElementHeatPropagator_mex.c", "getDownsampledParamVals_mex.c"):
         src = os.path.join(mexdir, name)
         if not os.path.exists(src):
             continue

@@ -15,6 +15,7 @@ from mexdriver import MexLib, run_heat_mex, run_mc_mex
 HERE = os.path.dirname(os.path.abspath(__file__))
 MC_MEX = os.path.join(HERE, "_build", "libMCmatlab_mex_shim.so")
 HEAT_MEX = os.path.join(HERE, "_build", "libfiniteElementHeatPropagator_mex_shim.so")
+BIN_MEX = os.path.join(HERE, "_build", "libgetDownsampledParamVals_mex_shim.so")
 built = pytest.mark.skipif(not (os.path.exists(MC_MEX) and os.path.exists(HEAT_MEX)), reason="MEX gateways not built")
 
 
@@ -88,3 +89,47 @@ def test_heat_gateway_equals_api():
     np.testing.assert_array_equal(r["Omega"], Om)
     np.testing.assert_array_equal(r["sensorTemps"], sens)
     np.testing.assert_array_equal(r["maxMediaTemps"].ravel(), mx.ravel())
+
+
+def _bin_mex(m, pv, n2):
+    from mexdriver import mxSINGLE
+
+    a = m.num(np.asarray(pv, np.float32), mxSINGLE, shape=np.asarray(pv).shape)
+    b = m.num(float(n2))
+    try:
+        outs = m.call(2, [a, b])
+        res = m.array(outs[0]), m.array(outs[1])
+        for o in outs:
+            m.lib.mxs_destroy(o)
+        return res
+    finally:
+        m.lib.mxs_destroy(a)
+        m.lib.mxs_destroy(b)
+
+
+@pytest.mark.skipif(not os.path.exists(BIN_MEX), reason="binning gateway not built")
+def test_binning_gateway_checks_arguments_then_reports_missing_device():
+    m = MexLib(BIN_MEX)
+    with pytest.raises(RuntimeError, match="MCmatlab:invalidInput"):
+        _bin_mex(m, np.zeros((2, 4), np.float32), 3)
+    if not _no_gpu():
+        pytest.skip("a device is visible")
+    with pytest.raises(RuntimeError, match="MCmatlab:GPU"):
+        _bin_mex(m, np.random.default_rng(0).random((3, 50)), 4)
+
+
+@pytest.mark.skipif(not os.path.exists(BIN_MEX), reason="binning gateway not built")
+@pytest.mark.gpu
+def test_binning_gateway_equals_reference_vectors():
+    """The gateway called exactly as the reference MEX file is (same driver,
+    same mxArrays) returns the reference build's outputs, classes and sizes."""
+    from test_oracle_binning import golden_cases
+
+    m = MexLib(BIN_MEX)
+    for name, pv, n2, out, bins in golden_cases():
+        o, b = _bin_mex(m, pv, n2)
+        assert o.dtype == np.float32 and o.shape == (3, n2), name
+        assert b.dtype == np.uint8 and b.shape == (1, pv.shape[1]), name
+        assert np.array_equal(o.view(np.uint32), out.view(np.uint32)), name
+        assert np.array_equal(b.reshape(-1), bins), name
+
