@@ -20,7 +20,7 @@ line (see the keys below), including:
 
 Usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--packets N] [--copies R]
        python bench.py --copy-probe BYTES   (plain copy kernel for counter scaling)
-       python bench.py --workload spectral64|heat200|heat200explicit|grad128|adjoint128|sweep32   (secondary lines)
+       python bench.py --workload spectral64|heat200|heat200explicit|grad128|adjoint128|sweep32|binning200   (secondary lines)
 """
 from __future__ import annotations
 
