@@ -11,6 +11,9 @@ produces in MATLAB before calling the MEX files.
   limit (simulateHeatDistribution.m:469-514).
 - sensor_weights: tempSensorCornerIdxs / tempSensorInterpWeights
   (simulateHeatDistribution.m:543-609).
+- bin_medium: the sub-media of an FR/T/FD-dependent medium
+  (getOpticalMediaProperties.m:148-163, getThermalMediaProperties.m:75-84):
+  the GPU binning followed by the callers' sortrows relabelling.
 """
 from __future__ import annotations
 
@@ -92,3 +95,25 @@ def sensor_weights(positions, x, y, z):
             w[i, a] = ww
         li[i] = idx[0] + idx[1] * axes[0].size + idx[2] * axes[0].size * axes[1].size
     return li, w
+
+
+def bin_medium(paramVals, nBins, device=0):
+    """Sub-media of one FR/T/FD-dependent medium as the MATLAB callers build
+    them (getOpticalMediaProperties.m:148-163, getThermalMediaProperties.m:
+    75-84): getDownsampledParamVals on the GPU, then the nBins downsampled
+    sets sorted as `sortrows` does (ascending by the first row, then the
+    second, then the third; equal sets keep their order) and every voxel's
+    label renumbered to the sorted order.  paramVals is 3 x N1 (mua, mus, g
+    or VHC, TC, 1) single; returns (sorted sets 3 x nBins single, labels 1..
+    nBins per set, int); the medium's voxels then get M = firstSubmedium +
+    label - 1."""
+    from . import api
+
+    if not 1 <= int(nBins) <= 255:
+        raise ValueError("nBins must be 1..255 (the labels are uint8)")
+    ds, b = api.getDownsampledParamVals(paramVals, int(nBins), device)
+    order = np.lexsort((ds[2], ds[1], ds[0]))  # stable, first row most significant
+    rank = np.empty(int(nBins), np.int64)
+    rank[order] = np.arange(1, int(nBins) + 1)
+    return ds[:, order], rank[b.astype(np.int64) - 1]
+

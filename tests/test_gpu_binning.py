@@ -111,3 +111,21 @@ def test_gpu_binning_twice_in_a_row(api):
     assert same(r1[0], r3[0]) and same(r1[1], r3[1])
     ro, rb = OL.oracle_binning(c, 9)
     assert same(r2[0], ro) and same(r2[1], rb)
+
+
+def test_gpu_bin_medium_sorts_and_relabels(api):
+    """The callers' sortrows step: sets ascending lexicographically (mua,
+    then mus, then g), and every label still names the same set."""
+    from mcmatlab_amd.media import bin_medium
+    from mcmatlab_amd.workloads import thermochromic_params
+
+    pv = thermochromic_params(n=40)
+    raw, rb = api.getDownsampledParamVals(pv, 30)
+    ds, lab = bin_medium(pv, 30)
+    keys = [tuple(c) for c in ds.T]
+    assert keys == sorted(keys)
+    assert sorted(map(tuple, raw.T)) == keys
+    assert np.array_equal(ds[:, lab - 1], raw[:, rb.astype(np.int64) - 1])
+    with pytest.raises(ValueError):
+        bin_medium(pv, 300)
+
