@@ -879,7 +879,7 @@ inline void with_wave_shape(WaveShape ws, F&& f) {
 // each plane of T and M (with its x/y halo) is staged through LDS once, so
 // every voxel is loaded from memory once and its four in-plane neighbours
 // come from LDS; the z neighbours are carried in registers.
-constexpr int kTileX = 32, kTileY = 8, kZRun = 16;
+constexpr int kTileX = 64, kTileY = 4, kZRun = 16;
 
 template <typename RT, bool LDS_COEF>
 __global__ void __launch_bounds__(kTileX * kTileY) heat_explicit_tile(HeatArgs<RT> a) {
