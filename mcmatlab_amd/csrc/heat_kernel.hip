@@ -1300,9 +1300,11 @@ __global__ void __launch_bounds__(kHeatTPB) heat_adj_combine(HeatArgs<double> a,
   }
   const double* __restrict__ cT = LDS_COEF ? sC : a.c;
   const int nx = a.nx, ny = a.ny, nz = a.nz, nxy = nx * ny;
-  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= (int64_t)nxy * nz) return;
-  const int x = (int)(i % nx), y = (int)((i / nx) % ny), z = (int)(i / nxy);
+  // one z plane per grid row: 32-bit in-plane index arithmetic
+  const int p = blockIdx.x * blockDim.x + threadIdx.x, z = blockIdx.y;
+  if (p >= nxy) return;
+  const int x = p % nx, y = p / nx;
+  const int64_t i = (int64_t)z * nxy + p;
   const int m = a.M[i];
   const double dt = a.dt;
   // L_d^T (D w) at i for w = mu1, mu2, mu3
@@ -2052,6 +2054,7 @@ int mcm_heat_adjoint_run(const mcm_heat_input* in, const mcm_heat_segment* segme
   out->J = J;
   if (out->Omega_f64) std::memcpy(out->Omega_f64, om.data(), bytes);
   if (!out->dJ_dQ && !out->dJ_dT0) return MCM_OK;
+  if (P->nz > 65535) return fail(MCM_ERR_UNSUPPORTED, "heat adjoint: nz > 65535 z planes");
   // reverse sweep, one checkpoint segment at a time
   HeatBuf seg, lam, lamN, mu1, mu2, mu3, dQ;
   if ((rc = halloc(seg, (size_t)(C + 1) * bytes)) || (rc = halloc(lam, bytes)) || (rc = halloc(lamN, bytes)) || (rc = halloc(mu1, bytes)) ||
@@ -2082,11 +2085,12 @@ int mcm_heat_adjoint_run(const mcm_heat_input* in, const mcm_heat_segment* segme
       adj_solve<2>(P, a, (const double*)lam.p, (double*)mu3.p, B, gz);
       adj_solve<1>(P, a, (const double*)mu3.p, (double*)mu2.p, B, gy);
       adj_solve<0>(P, a, (const double*)mu2.p, (double*)mu1.p, B, gx);
+      const dim3 gc((unsigned)((P->nx * P->ny + tpb - 1) / tpb), (unsigned)P->nz);
       if (P->nM * P->nM * 3 <= kTileCoef)
-        hipLaunchKernelGGL(heat_adj_combine<true>, dim3(gv), dim3(tpb), 0, s, a, (const double*)mu1.p, (const double*)mu2.p,
+        hipLaunchKernelGGL(heat_adj_combine<true>, gc, dim3(tpb), 0, s, a, (const double*)mu1.p, (const double*)mu2.p,
                            (const double*)mu3.p, x0, x1, (const uint8_t*)dMask.p, (double*)lamN.p, (double*)dQ.p, lights[(size_t)n]);
       else
-        hipLaunchKernelGGL(heat_adj_combine<false>, dim3(gv), dim3(tpb), 0, s, a, (const double*)mu1.p, (const double*)mu2.p,
+        hipLaunchKernelGGL(heat_adj_combine<false>, gc, dim3(tpb), 0, s, a, (const double*)mu1.p, (const double*)mu2.p,
                            (const double*)mu3.p, x0, x1, (const uint8_t*)dMask.p, (double*)lamN.p, (double*)dQ.p, lights[(size_t)n]);
       MCM_HIP(hipGetLastError());
       std::swap(lam.p, lamN.p);
