@@ -220,6 +220,56 @@ def getDownsampledParamVals(paramVals, N2, device=0):
     return out[:3 * n2].reshape(3, n2, order="F"), bins[:n1]
 
 
+def fr_dependent_MCmatlab(model, medium, params_of_FR, nBins, nIterations, seed=None):
+    """The fluence-rate-dependent iteration of runMonteCarlo.m:80-113 for one
+    FR-dependent medium (single wavelength, simType 1), around the GPU
+    operators: run i of nIterations launches nPhotonsRequested / 2^(nIterations
+    - i) packets; before each run the medium's voxels get their (mua, mus, g)
+    from the current fluence rate FR = P * NFR (zero before the first run)
+    through params_of_FR(FR) -> (mua, mus, g) arrays, are binned into nBins
+    sub-media (getDownsampledParamVals on the GPU with the callers' sortrows
+    relabelling, getOpticalMediaProperties.m:148-163), and the media table and
+    media map are rebuilt with those sub-media in the medium's place (later
+    media shift by nBins - 1; sub-media keep the medium's n and CDFidx).
+    `medium` is 1-based as in model.MC.M.  Returns the last run's model, whose
+    MC.M and MC.mediaProperties hold the split media."""
+    from .media import bin_medium
+
+    M0 = np.asarray(model.MC.M)
+    mp0 = {k: np.asarray(v, dtype=np.float64).reshape(np.asarray(v).shape[0], -1)[:, :1]
+           for k, v in model.MC.mediaProperties.items()}
+    nM = mp0["mua"].shape[0]
+    if not 1 <= medium <= nM:
+        raise ValueError("medium out of range")
+    if nM - 1 + nBins > 255:
+        raise ValueError("more than 255 media after splitting")
+    vox = (M0 == medium).ravel(order="F")
+    if not vox.any():
+        raise ValueError("the medium has no voxels")
+    nPh = float(model.MC.nPhotonsRequested)
+    FR = np.zeros(int(vox.sum()))
+    res = None
+    for i in range(1, int(nIterations) + 1):
+        mua, mus, g = (np.broadcast_to(np.asarray(v, np.float64), FR.shape) for v in params_of_FR(FR))
+        ds, lab = bin_medium(np.vstack([mua, mus, g]).astype(np.float32), nBins)
+        mp = {}
+        for k, col in mp0.items():
+            sub = ds[{"mua": 0, "mus": 1, "g": 2}[k]].astype(np.float64)[:, None] if k in ("mua", "mus", "g") \
+                else np.repeat(col[medium - 1:medium], nBins, axis=0)
+            mp[k] = np.concatenate([col[:medium - 1], sub, col[medium:]], axis=0)
+        Mf = M0.ravel(order="F").astype(np.int64)
+        Mf = np.where(Mf > medium, Mf + nBins - 1, Mf)
+        Mf[vox] = medium + lab - 1
+        m = model.copy()
+        m.MC.M = Mf.reshape(M0.shape, order="F").astype(np.uint8)
+        m.MC.mediaProperties = mp
+        m.MC.nPhotonsRequested = nPh * 0.5 ** (int(nIterations) - i)
+        res = MCmatlab(m, 1, seed=None if seed is None else seed + i)
+        if i < nIterations:
+            FR = float(model.MC.P) * np.asarray(res.MC.NFR).ravel(order="F")[vox]
+    return res
+
+
 def heat_adjoint(T, Omega, P, segments=None, targetMask=None, checkpointEvery=0, grads=True, T_f64=None, Omega_f64=None):
     """mcm_heat_adjoint_run: final damage J summed over the target voxels
     (targetMask, default: media with E != 0) after the schedule `segments`
