@@ -19,6 +19,10 @@ arguments and outputs:
       getThermalMediaProperties.m:80) -- farthest-point binning of a
       medium's per-voxel parameter sets on the GPU.
 
+  model = fr_dependent_MCmatlab(model, medium, params_of_FR, nBins, nIterations)
+      (runMonteCarlo.m:80-113) -- the fluence-rate-dependent iteration for
+      one medium: re-binned sub-media before every run, packet ramp.
+
 Errors raise MCMError (the MEX files raise MATLAB errors via
 mexErrMsgIdAndTxt).  There is no CPU fallback.
 """
