@@ -23,6 +23,10 @@ arguments and outputs:
       (runMonteCarlo.m:80-113) -- the fluence-rate-dependent iteration for
       one medium: re-binned sub-media before every run, packet ramp.
 
+  T, Omega, M = t_dependent_heat(T, Omega, P, M, VHC, TC, d, medium, thermal_of_T, nBins, updates)
+      (simulateHeatDistribution.m:515-541) -- heat updates with one
+      temperature-dependent medium re-binned before each update.
+
 Errors raise MCMError (the MEX files raise MATLAB errors via
 mexErrMsgIdAndTxt).  There is no CPU fallback.
 """
@@ -272,6 +276,60 @@ def fr_dependent_MCmatlab(model, medium, params_of_FR, nBins, nIterations, seed=
         if i < nIterations:
             FR = float(model.MC.P) * np.asarray(res.MC.NFR).ravel(order="F")[vox]
     return res
+
+
+def t_dependent_heat(T, Omega, P, M, VHC, TC, d, medium, thermal_of_T, nBins, updates, Q_abs=None):
+    """Heat simulation with a temperature-dependent medium, as
+    simulateHeatDistribution.m runs it: the schedule is cut into updates
+    (steps, dt, lightsOn), each one call of the heat operator continuing the
+    FP64 state, and before every update the medium's voxels get (VHC, TC)
+    from their current temperature through thermal_of_T(T) -> (VHC, TC)
+    arrays and are binned into nBins sub-media (GPU binning with the
+    callers' sortrows relabelling, getThermalMediaProperties.m:75-84); the
+    media map, dTdtperdeltaT (harmonic-mean conductances, per sub-medium
+    VHC) and dTdt_abs = Q_abs / VHC(voxel) are rebuilt from them
+    (heatSimParamsRecalc, simulateHeatDistribution.m:515-541).  M is the
+    1-based media map, VHC and TC the per-medium values of the other media,
+    d = (dx, dy, dz), Q_abs the absorbed power density per voxel [W/cm^3]
+    (None: no source); P supplies A, E (per medium), the boundary type and
+    the sensors.  `medium` is 1-based.  Returns (T, Omega, M_split) of the
+    final state (FP64 T and Omega) and the last media map (1-based)."""
+    from .media import bin_medium, heat_coefficients
+
+    M = np.asarray(M)
+    nM = len(VHC)
+    vox = (M == medium).ravel(order="F")
+    if not vox.any():
+        raise ValueError("the medium has no voxels")
+    A = np.asarray(P["A"], np.float64)
+    E = np.asarray(P["E"], np.float64)
+    Q = None if Q_abs is None else np.asarray(Q_abs, np.float64).ravel(order="F")
+    state = None
+    T_cur = np.asarray(T, np.float64).ravel(order="F")
+    Om_cur = Omega
+    Ms = M
+    for steps, dt, on in updates:
+        vhc, tc = (np.broadcast_to(np.asarray(v, np.float64), (int(vox.sum()),)) for v in thermal_of_T(T_cur[vox]))
+        ds, lab = bin_medium(np.vstack([vhc, tc, np.ones_like(vhc)]).astype(np.float32), nBins)
+        rep = lambda a, sub: np.concatenate([np.asarray(a, np.float64)[:medium - 1], sub, np.asarray(a, np.float64)[medium:]])
+        VHCs = rep(VHC, ds[0].astype(np.float64))
+        TCs = rep(TC, ds[1].astype(np.float64))
+        Mf = M.ravel(order="F").astype(np.int64)
+        Mf = np.where(Mf > medium, Mf + nBins - 1, Mf)
+        Mf[vox] = medium + lab - 1
+        Ms = Mf.reshape(M.shape, order="F")
+        Pu = dict(P)
+        Pu["M"] = (Ms - 1).astype(np.uint8)
+        Pu["A"] = rep(A, np.full(nBins, A[medium - 1]))
+        Pu["E"] = rep(E, np.full(nBins, E[medium - 1]))
+        Pu["dTdtperdeltaT"] = heat_coefficients(VHCs, TCs, *d)
+        vhc_vox = VHCs.astype(np.float32)[Mf - 1]
+        Pu["dTdt_abs"] = (np.zeros(Mf.size, np.float32) if Q is None else (Q / vhc_vox).astype(np.float32)).reshape(M.shape, order="F")
+        Pu["steps"], Pu["dt"], Pu["lightsOn"] = int(steps), float(dt), bool(on)
+        Tout, Om_cur, _, _ = finiteElementHeatPropagator(T, Om_cur, Pu, state_f64=state)
+        state = finiteElementHeatPropagator.last_state
+        T_cur = np.asarray(state[0], np.float64).ravel(order="F")
+    return state[0], state[1], Ms
 
 
 def heat_adjoint(T, Omega, P, segments=None, targetMask=None, checkpointEvery=0, grads=True, T_f64=None, Omega_f64=None):
