@@ -254,5 +254,6 @@ def thermochromic_params(n=200, seed=4):
     X, Y, Z = np.meshgrid(ax, ax, (np.arange(n) + 0.5) / n, indexing="ij")
     T = 37 + 40 * np.exp(-(X ** 2 + Y ** 2) / 0.02 - Z ** 2 / 0.08) + 0.2 * rng.standard_normal(X.shape)
     s = ((T - 37) / 40).ravel(order="F")
-    return np.vstack([0.5 + 9.5 * s, 200 - 60 * s, np.full(s.size, 0.9)]).astype(np.float32)
+    # column-major like the MATLAB array the MEX file receives (one set per column)
+    return np.asfortranarray(np.vstack([0.5 + 9.5 * s, 200 - 60 * s, np.full(s.size, 0.9)]).astype(np.float32))
 
