@@ -165,7 +165,7 @@ __global__ void __launch_bounds__(kBinTPB) bin_prepare(BinArgs a) {
 
 // Passes over the sets for ROWS varying rows: a thread takes kBinPer sets
 // kBinTPB apart and issues all their loads before the arithmetic.
-constexpr int kBinPer = 4;
+constexpr int kBinPer = 8;
 
 template <int ROWS>
 __device__ inline float sqr_dist_r(const float (&x)[ROWS > 0 ? ROWS : 1], const float (&p)[3]) {
