@@ -11,7 +11,8 @@
 // reduces them and selects the next pick on the device, so the N2 passes
 // run back to back on one stream with no host round trip.  (Selecting in
 // the last workgroup to finish, by an atomic ticket, serialised thousands
-// of same-address atomics per pass: 206 us per pass against 56 us.)
+// of same-address atomics per pass: 206 us per pass at 8e6 sets, against
+// 27 us for the pass and the separate selection.)
 //
 // Selection.  The reference's single-threaded pass scans k = 0..N1-1 and
 // keeps a running maximum whose index m starts at the previous pass's m
