@@ -875,14 +875,15 @@ inline void with_wave_shape(WaveShape ws, F&& f) {
 // Forward Euler step (no reference counterpart; the 7-point right-hand side
 // of substep 1a with all three axes fully weighted, the damage update and
 // the per-medium maxima of the DG-ADI step).  A workgroup owns a
-// kTileX x kTileY column tile and a run of kZRun z planes, marching up in z:
+// kTileX x kTileY column tile and a run of kZRun z planes (chosen per plan by
+// timing, see tune_explicit), marching up in z:
 // each plane of T and M (with its x/y halo) is staged through LDS once, so
 // every voxel is loaded from memory once and its four in-plane neighbours
 // come from LDS; the z neighbours are carried in registers.
-constexpr int kTileX = 64, kTileY = 4, kZRun = 16;
+constexpr int kTileX = 64, kTileY = 4;
 
 template <typename RT, bool LDS_COEF>
-__global__ void __launch_bounds__(kTileX * kTileY) heat_explicit_tile(HeatArgs<RT> a) {
+__global__ void __launch_bounds__(kTileX * kTileY) heat_explicit_tile(HeatArgs<RT> a, int kZRun) {
   __shared__ HeatLds<RT> L;
   __shared__ RT sT[kTileY + 2][kTileX + 2];
   __shared__ uint8_t sM[kTileY + 2][kTileX + 2];
@@ -1358,6 +1359,7 @@ struct mcm_heat_plan {
   int device = 0;
   hipStream_t stream = nullptr;
   int fp32 = 0, explicitScheme = 0;
+  int explicitZRun = 16;  // z planes per workgroup of the explicit stencil
   int nx = 0, ny = 0, nz = 0, nM = 0;
   int64_t L = 0;
   int sinked = 0, calcDamage = 0;
@@ -1418,6 +1420,13 @@ int halloc(HeatBuf& b, size_t bytes) {
 }
 
 template <typename RT>
+void launch_explicit(mcm_heat_plan* P, const HeatArgs<RT>& a, int zr) {
+  const unsigned ge = (unsigned)(((P->nx + kTileX - 1) / kTileX) * ((P->ny + kTileY - 1) / kTileY) * ((P->nz + zr - 1) / zr));
+  if (P->nM * P->nM * 3 <= kLdsCoef) hipLaunchKernelGGL((heat_explicit_tile<RT, true>), dim3(ge), dim3(kTileX * kTileY), 0, P->stream, a, zr);
+  else hipLaunchKernelGGL((heat_explicit_tile<RT, false>), dim3(ge), dim3(kTileX * kTileY), 0, P->stream, a, zr);
+}
+
+template <typename RT>
 HeatArgs<RT> make_args(mcm_heat_plan* P, double dt, int lightsOn) {
   HeatArgs<RT> a{};
   a.nx = P->nx;
@@ -1441,6 +1450,45 @@ HeatArgs<RT> make_args(mcm_heat_plan* P, double dt, int lightsOn) {
   a.Om = (RT*)P->Om.p;
   a.maxKey = (unsigned long long*)P->maxKey.p;
   return a;
+}
+
+// The explicit stencil's best run of z planes per workgroup depends on how
+// the workgroup count fills the GPU (measured, ms/step at 100^3 / 160^3 /
+// 200^3 / 256^3: 8 planes 0.0189 / 0.0446 / 0.0792 / 0.1272, 16 planes
+// 0.0232 / 0.0375 / 0.0628 / 0.1095, 32 planes 0.0409 / 0.0501 / 0.0561 /
+// 0.1175), so each explicit plan times a few candidate runs once at
+// creation.  The trial steps read the initial state and write only the
+// other state buffer (overwritten by the first real step), a scratch damage
+// field (the DG-ADI work buffer, unused by this scheme) and scratch maxima;
+// no sensor is recorded.
+template <typename RT>
+int tune_explicit(mcm_heat_plan* P) {
+  HeatArgs<RT> a = make_args<RT>(P, 1e-9, 1);
+  a.Om = (RT*)P->B.p;
+  HeatBuf keys;
+  if (int rc = halloc(keys, (size_t)P->nM * 8)) return rc;
+  a.maxKey = (unsigned long long*)keys.p;
+  hipEvent_t e0, e1;
+  MCM_HIP(hipEventCreate(&e0));
+  MCM_HIP(hipEventCreate(&e1));
+  float best = 0.f;
+  int bestZ = 16;
+  for (int zr : {8, 12, 16, 24, 32}) {
+    if (zr > 8 && zr >= P->nz + 8) break;  // longer runs than the grid change nothing
+    launch_explicit<RT>(P, a, zr);
+    (void)hipEventRecord(e0, P->stream);
+    for (int r = 0; r < 3; ++r) launch_explicit<RT>(P, a, zr);
+    (void)hipEventRecord(e1, P->stream);
+    float ms = 0.f;
+    if (hipEventSynchronize(e1) != hipSuccess || hipEventElapsedTime(&ms, e0, e1) != hipSuccess) break;
+    if (best == 0.f || ms < best) best = ms, bestZ = zr;
+  }
+  (void)hipEventDestroy(e0);
+  (void)hipEventDestroy(e1);
+  MCM_HIP(hipGetLastError());
+  MCM_HIP(hipStreamSynchronize(P->stream));
+  P->explicitZRun = bestZ;
+  return MCM_OK;
 }
 
 int ensure_sensor_capacity(mcm_heat_plan* P, int64_t cols) {
@@ -1548,9 +1596,7 @@ void enqueue_step(mcm_heat_plan* P, double dt, int lightsOn, const long long* co
     a.sCol = col;
   }
   if (P->explicitScheme) {
-    const unsigned ge = (unsigned)(((P->nx + kTileX - 1) / kTileX) * ((P->ny + kTileY - 1) / kTileY) * ((P->nz + kZRun - 1) / kZRun));
-    if (P->nM * P->nM * 3 <= kLdsCoef) hipLaunchKernelGGL((heat_explicit_tile<RT, true>), dim3(ge), dim3(kTileX * kTileY), 0, P->stream, a);
-    else hipLaunchKernelGGL((heat_explicit_tile<RT, false>), dim3(ge), dim3(kTileX * kTileY), 0, P->stream, a);
+    launch_explicit<RT>(P, a, P->explicitZRun);
     (void)tpb;
   } else {
     launch_sweep_x<RT>(P, a, (unsigned)((P->ny * P->nz + tpb - 1) / tpb));
@@ -1843,6 +1889,7 @@ int mcm_heat_plan_create(const mcm_heat_input* in, mcm_heat_plan** out) {
   rc = P->fp32 ? upload_state<float>(P.get(), in) : upload_state<double>(P.get(), in);
   if (rc) return rc;
   if ((rc = ensure_sensor_capacity(P.get(), std::max<int64_t>(in->steps, 1) + 1))) return rc;
+  if (P->explicitScheme && (rc = P->fp32 ? tune_explicit<float>(P.get()) : tune_explicit<double>(P.get()))) return rc;
   MCM_HIP(hipStreamSynchronize(s));
   *out = P.release();
   return MCM_OK;
