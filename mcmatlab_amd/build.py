@@ -79,8 +79,8 @@ def build_library(verbose=True):
 
 
 def build_oracle(verbose=True):
-    """The CPU oracle (test infrastructure): the port always; the reference
-    build only where the reference sources exist."""
+    """The CPU oracle (test infrastructure): the restatement always; the
+    reference's Lambert W source only where the reference sources exist."""
     r = subprocess.run(["make", "-s", "-C", os.path.join(ROOT, "oracle"), "all"], capture_output=True, text=True)
     if r.returncode != 0:
         sys.stderr.write(r.stdout + r.stderr)
@@ -101,7 +101,7 @@ def build_test_harnesses(verbose=True):
         if verbose:
             print("  built tests/_build/libmc_host.so")
     mexdir = os.path.join(PKG, "mex")
-    shim = os.path.join(ROOT, "oracle", "mexshim")
+    shim = os.path.join(ROOT, "tests", "mexshim")
     for name in ("MCmatlab_mex.c", "finiteElementHeatPropagator_mex.c", "getDownsampledParamVals_mex.c"):
         src = os.path.join(mexdir, name)
         if not os.path.exists(src):

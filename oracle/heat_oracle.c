@@ -34,7 +34,15 @@ typedef struct heat_ctx {
   long nx, ny, nz, nM;
   const unsigned char *M;
   bool sinked, lightsOn, calcDamage;
+  int nthreads;
 } heat_ctx;
+
+#ifdef _OPENMP
+#include <omp.h>
+static inline int omp_tid(void) { return omp_get_thread_num(); }
+#else
+static inline int omp_tid(void) { return 0; }
+#endif
 
 static inline bool frozen(const heat_ctx *H, long ix, long iy, long iz) {
   return H->sinked && (ix == 0 || ix == H->nx - 1 || iy == 0 || iy == H->ny - 1 || iz == 0 || iz == H->nz - 1);
@@ -57,9 +65,12 @@ static inline bool frozen(const heat_ctx *H, long ix, long iy, long iz) {
 #undef EXPF
 
 int oracle_heat_run(const mcm_heat_input *in, mcm_heat_output *out, int variant, int nthreads) {
-  (void)nthreads;
   if (!in || !out || !in->T || !in->M || !in->dTdtperdeltaT || !in->dTdt_abs) return MCM_ERR_INVALID_ARGUMENT;
-  heat_ctx H = {in->nx, in->ny, in->nz, in->nM, in->M, in->heatBoundaryType != 0, in->lightsOn != 0, in->Omega != NULL};
+  heat_ctx H = {in->nx, in->ny, in->nz, in->nM, in->M, in->heatBoundaryType != 0, in->lightsOn != 0, in->Omega != NULL,
+                nthreads > 1 ? nthreads : 1};
+#ifndef _OPENMP
+  H.nthreads = 1;
+#endif
   if (variant == ORC_HEAT_FP32_DGADI) return heat_run_f32(&H, in, out, 0);
   if (variant == ORC_HEAT_FP64_DGADI) return heat_run_f64(&H, in, out, 0);
   if (variant == ORC_HEAT_FP64_EXPLICIT) return heat_run_f64(&H, in, out, 1);

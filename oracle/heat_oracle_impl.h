@@ -13,6 +13,7 @@
 static void FN(sub1a)(const heat_ctx *H, const float *c, const float *Q, RT dt, const RT *T1, RT *T2) {
   const long nx = H->nx, ny = H->ny, nz = H->nz, nM = H->nM, nxy = nx * ny;
   const unsigned char *M = H->M;
+#pragma omp parallel for num_threads(H->nthreads) if (H->nthreads > 1)
   for (long iz = 0; iz < nz; iz++)
     for (long iy = 0; iy < ny; iy++)
       for (long ix = 0; ix < nx; ix++) {
@@ -70,6 +71,7 @@ static void FN(sub_corr)(const heat_ctx *H, const float *c, RT dt, int d, const 
   const unsigned char *M = H->M;
   const long st = d == 1 ? nx : nxy;
   const long nd = d == 1 ? ny : nz;
+#pragma omp parallel for num_threads(H->nthreads) if (H->nthreads > 1)
   for (long iz = 0; iz < nz; iz++)
     for (long iy = 0; iy < ny; iy++)
       for (long ix = 0; ix < nx; ix++) {
@@ -96,23 +98,33 @@ static void FN(step_explicit)(const heat_ctx *H, const float *c, const float *Q,
                               const RT *T1, RT *T2, RT *Omega, RT *maxT) {
   const long nx = H->nx, ny = H->ny, nz = H->nz, nM = H->nM, nxy = nx * ny;
   const unsigned char *M = H->M;
-  for (long iz = 0; iz < nz; iz++)
-    for (long iy = 0; iy < ny; iy++)
-      for (long ix = 0; ix < nx; ix++) {
-        const long i = ix + iy * nx + iz * nxy;
-        RT dTdt = 0;
-        if (!frozen(H, ix, iy, iz)) {
-          dTdt = H->lightsOn ? (RT)Q[i] : 0;
-          if (ix != 0) dTdt += (T1[i - 1] - T1[i]) * CC(M[i], M[i - 1], 0);
-          if (ix != nx - 1) dTdt += (T1[i + 1] - T1[i]) * CC(M[i], M[i + 1], 0);
-          if (iy != 0) dTdt += (T1[i - nx] - T1[i]) * CC(M[i], M[i - nx], 1);
-          if (iy != ny - 1) dTdt += (T1[i + nx] - T1[i]) * CC(M[i], M[i + nx], 1);
-          if (iz != 0) dTdt += (T1[i - nxy] - T1[i]) * CC(M[i], M[i - nxy], 2);
-          if (iz != nz - 1) dTdt += (T1[i + nxy] - T1[i]) * CC(M[i], M[i + nxy], 2);
+#pragma omp parallel num_threads(H->nthreads) if (H->nthreads > 1)
+  {
+    RT mx[256];
+    for (long m = 0; m < nM; m++) mx[m] = maxT[m];
+#pragma omp for
+    for (long iz = 0; iz < nz; iz++)
+      for (long iy = 0; iy < ny; iy++)
+        for (long ix = 0; ix < nx; ix++) {
+          const long i = ix + iy * nx + iz * nxy;
+          RT dTdt = 0;
+          if (!frozen(H, ix, iy, iz)) {
+            dTdt = H->lightsOn ? (RT)Q[i] : 0;
+            if (ix != 0) dTdt += (T1[i - 1] - T1[i]) * CC(M[i], M[i - 1], 0);
+            if (ix != nx - 1) dTdt += (T1[i + 1] - T1[i]) * CC(M[i], M[i + 1], 0);
+            if (iy != 0) dTdt += (T1[i - nx] - T1[i]) * CC(M[i], M[i - nx], 1);
+            if (iy != ny - 1) dTdt += (T1[i + nx] - T1[i]) * CC(M[i], M[i + nx], 1);
+            if (iz != 0) dTdt += (T1[i - nxy] - T1[i]) * CC(M[i], M[i - nxy], 2);
+            if (iz != nz - 1) dTdt += (T1[i + nxy] - T1[i]) * CC(M[i], M[i + nxy], 2);
+          }
+          T2[i] = T1[i] + dt * dTdt;
+          FN(damage_and_max)(H, i, dt, logA, E, Rgas, czero, T1, T2, Omega, mx);
         }
-        T2[i] = T1[i] + dt * dTdt;
-        FN(damage_and_max)(H, i, dt, logA, E, Rgas, czero, T1, T2, Omega, maxT);
-      }
+    /* a maximum does not depend on the order it is taken in */
+#pragma omp critical(orc_heat_max)
+    for (long m = 0; m < nM; m++)
+      if (maxT[m] < mx[m]) maxT[m] = mx[m];
+  }
 }
 
 /* recordTempSensors, :652-663 */
@@ -129,7 +141,7 @@ static void FN(sensors)(const heat_ctx *H, const mcm_heat_input *in, const RT *T
   }
 }
 
-/* mexFunction, :688-821, single-threaded */
+/* mexFunction, :688-821; on H->nthreads threads (the values do not depend on it) */
 static int FN(heat_run)(const heat_ctx *H, const mcm_heat_input *in, mcm_heat_output *out, int explicit_scheme) {
   const long nx = H->nx, ny = H->ny, nz = H->nz, nM = H->nM, L = nx * ny * nz, nxy = nx * ny;
   const long nt = in->steps;
@@ -140,9 +152,10 @@ static int FN(heat_run)(const heat_ctx *H, const mcm_heat_input *in, mcm_heat_ou
   RT *T1 = (RT *)malloc(L * sizeof(RT));
   RT *T2 = (RT *)malloc(L * sizeof(RT));
   RT *Om = (RT *)malloc((H->calcDamage ? L : 1) * sizeof(RT));
-  long bl = nx * nz > ny ? nx * nz : ny;
-  if (bl < nx) bl = nx;
-  RT *b = (RT *)malloc(bl * sizeof(RT));
+  /* one scratch row per thread, as long as the longest line */
+  long bl = nx > ny ? nx : ny;
+  if (bl < nz) bl = nz;
+  RT *b = (RT *)malloc(bl * (H->nthreads > 1 ? H->nthreads : 1) * sizeof(RT));
   RT logA[256], E[256], maxT[256];
   if (!T1 || !T2 || !Om || !b || nM > 256) {
     free(T1); free(T2); free(Om); free(b);
@@ -165,28 +178,48 @@ static int FN(heat_run)(const heat_ctx *H, const mcm_heat_input *in, mcm_heat_ou
     } else {
       const float *c = in->dTdtperdeltaT;
       FN(sub1a)(H, c, in->dTdt_abs, dt, T1, T2);
-      for (long iz = 0; iz < nz; iz++)
+      /* every line solve is independent of the others, so lines may run on
+       * several threads (each with its own scratch row of b) */
+#pragma omp parallel for num_threads(H->nthreads) if (H->nthreads > 1)
+      for (long iz = 0; iz < nz; iz++) {
+        RT *bt = b + bl * omp_tid();
         for (long iy = 0; iy < ny; iy++) {
           long pos[3] = {0, iy, iz};
-          FN(thomas_line)(H, c, dt, 0, iy * nx + iz * nxy, 1, nx, T2, b, 1, pos);
+          FN(thomas_line)(H, c, dt, 0, iy * nx + iz * nxy, 1, nx, T2, bt, 1, pos);
         }
+      }
       FN(sub_corr)(H, c, dt, 1, T1, T2);
-      for (long iz = 0; iz < nz; iz++)
+#pragma omp parallel for num_threads(H->nthreads) if (H->nthreads > 1)
+      for (long iz = 0; iz < nz; iz++) {
+        RT *bt = b + bl * omp_tid();
         for (long ix = 0; ix < nx; ix++) {
           long pos[3] = {ix, 0, iz};
-          FN(thomas_line)(H, c, dt, 1, ix + iz * nxy, nx, ny, T2, b, 1, pos);
+          FN(thomas_line)(H, c, dt, 1, ix + iz * nxy, nx, ny, T2, bt, 1, pos);
         }
+      }
       FN(sub_corr)(H, c, dt, 2, T1, T2);
       /* the reference sweeps z with x innermost (:615-645); every z line is
        * independent, so line-at-a-time order gives the same values */
-      for (long iy = 0; iy < ny; iy++)
+#pragma omp parallel for num_threads(H->nthreads) if (H->nthreads > 1)
+      for (long iy = 0; iy < ny; iy++) {
+        RT *bt = b + bl * omp_tid();
         for (long ix = 0; ix < nx; ix++) {
           long pos[3] = {ix, iy, 0};
-          FN(thomas_line)(H, c, dt, 2, ix + iy * nx, nxy, nz, T2, b, 1, pos);
+          FN(thomas_line)(H, c, dt, 2, ix + iy * nx, nxy, nz, T2, bt, 1, pos);
         }
-      for (long iy = 0; iy < ny; iy++)
-        for (long iz = nz - 1; iz >= 0; iz--)
-          for (long ix = 0; ix < nx; ix++) FN(damage_and_max)(H, ix + iy * nx + iz * nxy, dt, logA, E, Rgas, czero, T1, T2, Om, maxT);
+      }
+#pragma omp parallel num_threads(H->nthreads) if (H->nthreads > 1)
+      {
+        RT mx[256];
+        for (long m = 0; m < nM; m++) mx[m] = maxT[m];
+#pragma omp for
+        for (long iy = 0; iy < ny; iy++)
+          for (long iz = nz - 1; iz >= 0; iz--)
+            for (long ix = 0; ix < nx; ix++) FN(damage_and_max)(H, ix + iy * nx + iz * nxy, dt, logA, E, Rgas, czero, T1, T2, Om, mx);
+#pragma omp critical(orc_heat_max)
+        for (long m = 0; m < nM; m++)
+          if (maxT[m] < mx[m]) maxT[m] = mx[m];
+      }
     }
     RT *tmp = T1;
     T1 = T2;

@@ -289,6 +289,8 @@ static real lambert_wm1(real x) {
   return w;
 }
 
+double oracle_lambert_wm1(double x) { return (double)lambert_wm1((real)x); }
+
 /* ------------------------------------------------------------ paths */
 
 /* addToPhotonPath, MCmatlablib.c:478-502 (CPU branch) */

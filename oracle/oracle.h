@@ -60,6 +60,9 @@ void oracle_philox4x32_10(const uint32_t ctr[4], const uint32_t key[2], uint32_t
 /* dSFMT (0,1] stream, exposed for the known-answer test. */
 void oracle_dsfmt_stream(uint32_t seed, int n, double *out);
 const char *oracle_last_error(void);
+/* The Lambert W lower branch the LG01 source samples with, exposed for its
+ * known-answer test against the reference's lambert.c. */
+double oracle_lambert_wm1(double x);
 
 #ifdef __cplusplus
 }

@@ -209,6 +209,9 @@ extern "C" int mc_host_run(const mcm_mc_input* in, mcm_mc_output* out, uint64_t 
   return MCM_OK;
 }
 
+// The FP32 Lambert W lower branch of the device code (LG01 source sampling).
+extern "C" float mc_host_lambert_wm1(float x) { return lambert_wm1(x); }
+
 // Philox words of the device header (for the RNG known-answer test).
 extern "C" void mcm_test_philox(const uint32_t* ctr, const uint32_t* key, uint32_t* out) {
   uint32_t c[4] = {ctr[0], ctr[1], ctr[2], ctr[3]};

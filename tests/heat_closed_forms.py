@@ -15,6 +15,21 @@ in FP64.
                     then (. - dt lz/2) / (1 - dt lz/2).
   frozen            fixed-temperature boundary voxels never change.
   arrhenius         constant T: Omega = n dt A exp(-E / (R (T + 273.15))).
+  arrhenius_midpoint one step of a decaying eigenmode on a 60 degC offset:
+                    Omega = dt A exp(-E / (R ((T^n + T^(n+1)) / 2 + 273.15)))
+                    per voxel, with T^(n+1) from the eigenmode's closed form
+                    (the damage integrand is taken at the step's mean
+                    temperature, not at either end).
+  steady_series     three media in slabs along x between two fixed-temperature
+                    ends: the series-conductance profile
+                    T_k = T_0 + (T_N - T_0) sum_{j<k} 1/G_j / sum_j 1/G_j,
+                    G_j = 2 TC_a TC_b / (TC_a + TC_b) of the face between
+                    voxels j and j+1, is a steady state of the heterogeneous
+                    operator (flux continuity), for any VHC of the media;
+                    a coefficient read with its media swapped breaks it.
+  sensors_linear    a field linear in x, y and z on a homogeneous grid with
+                    fixed-temperature boundaries is steady, and trilinear
+                    sensors read it exactly at their positions.
 """
 from __future__ import annotations
 
@@ -113,3 +128,75 @@ def arrhenius(run):
     want = steps * dt * A * math.exp(-E / (8.3144598 * (70.0 + 273.15)))
     np.testing.assert_allclose(r["Omega_f64"], want, rtol=1e-12)
     np.testing.assert_allclose(r["maxMediaTemps"], 70.0, rtol=1e-6)
+
+
+def arrhenius_midpoint(run, scheme):
+    n = (12, 10, 8)
+    d = 0.005
+    kk = (1, 2, 1)
+    A, E = 7.6e66, 422.5e3
+    dt = 1e-3 if scheme == "explicit" else 5e-2
+    P = _params(n, d=d, steps=1, dt=dt, A=(A,), E=(E,))
+    c = float(np.asarray(P["dTdtperdeltaT"], dtype=np.float64)[0, 0, 0])
+    modes = [np.cos(math.pi * kk[a] * (np.arange(n[a]) + 0.5) / n[a]) for a in range(3)]
+    v = 4.0 * modes[0][:, None, None] * modes[1][None, :, None] * modes[2][None, None, :]
+    lam = [-4 * c * math.sin(math.pi * kk[a] / (2 * n[a])) ** 2 for a in range(3)]
+    if scheme == "explicit":
+        g = 1 + dt * sum(lam)
+    else:
+        lx, ly, lz = lam
+        g = (1 + dt * (lx / 2 + ly + lz)) / (1 - dt * lx / 2)
+        g = (g - dt * ly / 2) / (1 - dt * ly / 2)
+        g = (g - dt * lz / 2) / (1 - dt * lz / 2)
+    T0 = np.asfortranarray(60.0 + v)
+    r = run(T0.astype(np.float32), np.zeros(n, np.float32), P, T_f64=T0)
+    T1 = 60.0 + g * v
+    np.testing.assert_allclose(r["T_f64"], T1, rtol=0, atol=1e-11)
+    want = dt * A * np.exp(-E / (8.3144598 * ((T0 + T1) / 2 + 273.15)))
+    np.testing.assert_allclose(r["Omega_f64"], want, rtol=1e-9)
+    # the damage at either end of the step differs by far more than that
+    assert np.max(np.abs(dt * A * np.exp(-E / (8.3144598 * (T1 + 273.15))) / want - 1)) > 1e-4
+
+
+def steady_series(run, scheme):
+    nx = 14
+    n = (nx, 3, 3)
+    d = 0.005
+    VHC = np.array([3.7, 1.2, 4.1])
+    TC = np.array([0.0037, 0.0011, 0.0052])
+    media = np.array([0] * 4 + [1] * 5 + [2] * 5, np.uint8)
+    M = np.repeat(np.repeat(media[:, None, None], 3, axis=1), 3, axis=2).copy(order="F")
+    dt = 1e-4 if scheme == "explicit" else 1e-3
+    P = _params(n, VHC=VHC, TC=TC, M=M, bt=1, steps=20, dt=dt)
+    a, b = media[:-1], media[1:]
+    G = 2 * TC[a] * TC[b] / (TC[a] + TC[b])
+    R = np.concatenate([[0.0], np.cumsum(1 / G)])
+    prof = 20.0 + 30.0 * R / R[-1]
+    T0 = np.asfortranarray(np.repeat(np.repeat(prof[:, None, None], 3, axis=1), 3, axis=2))
+    r = run(T0.astype(np.float32), np.zeros(n, np.float32), P, T_f64=T0)
+    # the coefficient table is single precision: c[a,b] VHC[a] is symmetric to ~1e-7
+    np.testing.assert_allclose(r["T_f64"], T0, rtol=0, atol=2e-6 * 30)
+
+
+def sensors_linear(run, scheme):
+    from mcmatlab_amd.media import sensor_weights
+
+    n = (9, 8, 7)
+    d = 0.005
+    dt = 2e-4 if scheme == "explicit" else 1e-3
+    P = _params(n, d=d, bt=1, steps=4, dt=dt)
+    x = (np.arange(n[0]) - (n[0] - 1) / 2) * d
+    y = (np.arange(n[1]) - (n[1] - 1) / 2) * d
+    z = (np.arange(n[2]) + 0.5) * d
+    pos = np.array([[0.0031, -0.0047, 0.0122], [-0.011, 0.006, 0.021], [0.0, 0.0, 0.0125]])
+    li, w = sensor_weights(pos, x, y, z)
+    P["tempSensorCornerIdxs"], P["tempSensorInterpWeights"] = li, w
+    X, Y, Z = np.meshgrid(x, y, z, indexing="ij")
+    f = lambda px, py, pz: 40.0 + 300.0 * px - 200.0 * py + 500.0 * pz
+    T0 = np.asfortranarray(f(X, Y, Z))
+    r = run(T0.astype(np.float32), np.zeros(n, np.float32), P, T_f64=T0)
+    np.testing.assert_allclose(r["T_f64"], T0, rtol=0, atol=1e-11)
+    want = f(pos[:, 0], pos[:, 1], pos[:, 2])
+    S = np.asarray(r["sensorTemps"], np.float64).reshape(len(pos), -1, order="F")
+    assert S.shape[1] == 5
+    np.testing.assert_allclose(S, np.repeat(want[:, None], 5, axis=1), rtol=0, atol=1e-9)

@@ -1,6 +1,6 @@
-"""Drive a MEX-style shared library (the reference compiled against the
-project's mx-API shim, or this project's own MEX layer built the same way)
-from Python: model -> mxArray object tree -> mexFunction -> outputs.
+"""Drive this project's MEX layer (the gateways under mcmatlab_amd/mex, built
+against the project's mx-API shim in tests/mexshim) from Python:
+model -> mxArray object tree -> mexFunction -> outputs.
 
 Test infrastructure only.
 """
@@ -46,7 +46,6 @@ class MexLib:
         L.mxs_last_error.restype = C.c_char_p
         L.mxs_last_error_id.restype = C.c_char_p
         L.mxs_set_interrupt_after.argtypes = [C.c_int64]
-        L.mxs_set_clock_us.argtypes = [C.c_int64]
         self.mexFunction = C.cast(L.mexFunction, vp)
 
     # ---- construction
@@ -147,9 +146,8 @@ def _mc_object(m: MexLib, mc):
     })
 
 
-def run_mc_mex(m: MexLib, model: Model, simType=1, clock_us=0, interrupt_after=-1):
+def run_mc_mex(m: MexLib, model: Model, simType=1, interrupt_after=-1):
     """Call `model = MCmatlab(model, simType)` and return the output dict."""
-    m.lib.mxs_set_clock_us(int(clock_us))
     m.lib.mxs_set_interrupt_after(int(interrupt_after))
     G = m.obj({"dx": m.num(model.G.dx), "dy": m.num(model.G.dy), "dz": m.num(model.G.dz)})
     mc = model.FMC if simType == 2 else model.MC

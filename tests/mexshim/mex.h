@@ -2,12 +2,8 @@
  * mex.h -- a minimal, project-written stand-in for the subset of MATLAB's
  * MEX/matrix C API that MCmatlab's two MEX entry points use.
  *
- * TEST INFRASTRUCTURE ONLY.  It exists so that
- *   (1) the reference C sources (MCmatlab.c, finiteElementHeatPropagator.c)
- *       can be compiled in this container into oracle/_ref/ and driven from
- *       Python with pinned seeds, and
- *   (2) this project's own MEX layer (the sources under mcmatlab_amd/mex) can be exercised
- *       without MATLAB.
+ * TEST INFRASTRUCTURE ONLY.  It exists so that this project's own MEX layer
+ * (the gateways under mcmatlab_amd/mex) can be exercised without MATLAB.
  * It is never linked into the shipped library.
  *
  * Data model: every mxArray is a scalar numeric/logical array, or a scalar
@@ -120,8 +116,6 @@ const char *mxs_last_error_id(void);
 /* Make utIsInterruptPending() return true from its k-th call on (k<0: never). */
 void mxs_set_interrupt_after(int64_t k);
 int64_t mxs_interrupt_polls(void);
-/* Frozen monotonic clock for the reference's getMicroSeconds() (microseconds). */
-void mxs_set_clock_us(int64_t us);
 
 #ifdef __cplusplus
 }

@@ -36,7 +36,6 @@ static char g_err_id[256];
 static jmp_buf *g_jmp = NULL;
 static int64_t g_interrupt_after = -1;
 static int64_t g_interrupt_polls = 0;
-static int64_t g_clock_us = 0;
 
 static void fail(const char *msg) {
   snprintf(g_err, sizeof g_err, "%s", msg);
@@ -220,16 +219,6 @@ bool utIsInterruptPending(void) {
   return g_interrupt_after >= 0 && g_interrupt_polls >= g_interrupt_after;
 }
 
-/* Replacement for clock_gettime in the reference's getMicroSeconds(): the
- * reference seeds its RNG with the microsecond clock, so a frozen clock pins
- * the seed.  The reference is compiled with -Dclock_gettime=mxs_clock_gettime. */
-int mxs_clock_gettime(clockid_t clk, struct timespec *ts) {
-  (void)clk;
-  ts->tv_sec = (time_t)(g_clock_us / 1000000);
-  ts->tv_nsec = (long)((g_clock_us % 1000000) * 1000);
-  return 0;
-}
-
 /* ---- driver helpers ---- */
 mxArray *mxs_create_numeric(int classid, int ndim, const int64_t *dims, const void *data) {
   mwSize d[8];
@@ -283,4 +272,3 @@ const char *mxs_last_error(void) { return g_err; }
 const char *mxs_last_error_id(void) { return g_err_id; }
 void mxs_set_interrupt_after(int64_t k) { g_interrupt_after = k; }
 int64_t mxs_interrupt_polls(void) { return g_interrupt_polls; }
-void mxs_set_clock_us(int64_t us) { g_clock_us = us; }

@@ -1,5 +1,6 @@
 """Python access to the CPU oracle (oracle/_build/liboracle*.so) and to the
-reference build (oracle/_ref/*.so).  Test infrastructure only."""
+reference's Lambert W source compiled into oracle/_ref.  Test infrastructure
+only."""
 from __future__ import annotations
 
 import ctypes as C
@@ -13,10 +14,8 @@ ROOT = abi.repo_root()
 ORACLE_DIR = os.path.join(ROOT, "oracle")
 PORT_LIB = os.path.join(ORACLE_DIR, "_build", "liboracle.so")
 PORT_OMP_LIB = os.path.join(ORACLE_DIR, "_build", "liboracle_omp.so")
-REF_MC_LIB = os.path.join(ORACLE_DIR, "_ref", "libref_mcmatlab.so")
-REF_HEAT_LIB = os.path.join(ORACLE_DIR, "_ref", "libref_heat.so")
-REF_MC_OMP_LIB = os.path.join(ORACLE_DIR, "_ref", "libref_mcmatlab_omp.so")
-REF_BINNING_LIB = os.path.join(ORACLE_DIR, "_ref", "libref_binning.so")
+PORT_FAST_LIB = os.path.join(ORACLE_DIR, "_build", "liboracle_fast.so")
+REF_LAMBERT_LIB = os.path.join(ORACLE_DIR, "_ref", "libref_lambert.so")
 
 RNG_DSFMT = 0
 RNG_PHILOX = 1
@@ -48,8 +47,14 @@ def load_oracle(omp=False):
     return _libs[path]
 
 
-def ref_available():
-    return os.path.exists(REF_MC_LIB) and os.path.exists(REF_HEAT_LIB)
+def host_threads(cap=16):
+    """Threads for OpenMP oracle runs: the cores this process may use, at
+    most `cap`."""
+    try:
+        n = len(os.sched_getaffinity(0))
+    except AttributeError:
+        n = os.cpu_count() or 1
+    return max(1, min(cap, n))
 
 
 _libc = C.CDLL(None)
@@ -69,11 +74,13 @@ def oracle_mc(model, simType=1, seed=0, rng=RNG_DSFMT, nthreads=1, stats=False, 
     return res
 
 
-def oracle_heat(T, Omega, P, variant=HEAT_FP64_DGADI, want_f64=True, T_f64=None, Omega_f64=None):
-    lib = load_oracle()
+def oracle_heat(T, Omega, P, variant=HEAT_FP64_DGADI, want_f64=True, T_f64=None, Omega_f64=None, nthreads=1):
+    """nthreads > 1 runs the OpenMP build; every value is independent of the
+    thread count (lines and voxels are independent, maxima are exact)."""
+    lib = load_oracle(nthreads > 1)
     inp = abi.HeatInputs(T, Omega, P, T_f64=T_f64, Omega_f64=Omega_f64)
     out = abi.HeatOutputs(inp, want_f64=want_f64)
-    rc = lib.oracle_heat_run(C.byref(inp.struct), C.byref(out.struct), variant, 1)
+    rc = lib.oracle_heat_run(C.byref(inp.struct), C.byref(out.struct), variant, int(nthreads))
     if rc != 0:
         raise RuntimeError("oracle_heat_run failed (%d)" % rc)
     return out.result()
@@ -91,34 +98,6 @@ def oracle_binning(paramVals, n2):
     if rc != 0:
         raise RuntimeError("oracle_downsample_params failed (%d)" % rc)
     return out.reshape(3, n2, order="F"), bins
-
-
-def ref_binning(paramVals, n2):
-    """The same through the reference's getDownsampledParamVals.c compiled
-    from source against the mx shim (only where oracle/_ref is built)."""
-    from mexdriver import MexLib, mxSINGLE
-
-    m = _ref_binning_lib()
-    pv = np.asarray(paramVals, dtype=np.float32).reshape(3, -1)
-    a = m.num(pv, mxSINGLE, shape=pv.shape)
-    b = m.num(float(n2))
-    try:
-        outs = m.call(2, [a, b])
-        res = m.array(outs[0]), m.array(outs[1]).reshape(-1)
-        for o in outs:
-            m.lib.mxs_destroy(o)
-        return res
-    finally:
-        m.lib.mxs_destroy(a)
-        m.lib.mxs_destroy(b)
-
-
-def _ref_binning_lib():
-    from mexdriver import MexLib
-
-    if REF_BINNING_LIB not in _libs:
-        _libs[REF_BINNING_LIB] = MexLib(REF_BINNING_LIB)
-    return _libs[REF_BINNING_LIB]
 
 
 def philox(ctr, key):

@@ -1,14 +1,14 @@
 """GPU media binning (getDownsampledParamVals) through the C ABI: bit for bit
-the reference build's outputs (committed vectors) and the oracle's, on
-small and large inputs, with ties spread over many workgroups, rows that do
-not vary, more bins than distinct sets and labels past 255."""
+the hand-worked answers of the reference's algorithm and the oracle's
+outputs, on small and large inputs, with ties spread over many workgroups,
+rows that do not vary, more bins than distinct sets and labels past 255."""
 from __future__ import annotations
 
 import numpy as np
 import pytest
 
 import oracle_lib as OL
-from test_oracle_binning import golden_cases
+from test_oracle_binning import worked_cases
 
 pytestmark = pytest.mark.gpu
 
@@ -28,10 +28,10 @@ def same(a, b):
     return np.array_equal(a, b)
 
 
-@pytest.mark.parametrize("case", golden_cases(), ids=lambda c: c[0])
-def test_gpu_binning_matches_reference_vectors(api, case):
+@pytest.mark.parametrize("case", worked_cases(), ids=lambda c: c[0])
+def test_gpu_binning_matches_worked_cases(api, case):
     name, pv, n2, out, bins = case
-    print(f"[gpu-binning] golden {name}", flush=True)
+    print(f"[gpu-binning] worked case {name}", flush=True)
     o, b = api.getDownsampledParamVals(pv, n2)
     assert same(o, out)
     assert same(b, bins)

@@ -120,13 +120,14 @@ def test_binning_gateway_checks_arguments_then_reports_missing_device():
 
 @pytest.mark.skipif(not os.path.exists(BIN_MEX), reason="binning gateway not built")
 @pytest.mark.gpu
-def test_binning_gateway_equals_reference_vectors():
-    """The gateway called exactly as the reference MEX file is (same driver,
-    same mxArrays) returns the reference build's outputs, classes and sizes."""
-    from test_oracle_binning import golden_cases
+def test_binning_gateway_equals_worked_cases():
+    """The gateway called with the mxArrays the MATLAB callers pass returns
+    the worked answers of the reference's algorithm with the reference's
+    output classes and sizes."""
+    from test_oracle_binning import worked_cases
 
     m = MexLib(BIN_MEX)
-    for name, pv, n2, out, bins in golden_cases():
+    for name, pv, n2, out, bins in worked_cases():
         o, b = _bin_mex(m, pv, n2)
         assert o.dtype == np.float32 and o.shape == (3, n2), name
         assert b.dtype == np.uint8 and b.shape == (1, pv.shape[1]), name
