@@ -118,34 +118,121 @@ def roofline_counts():
         return json.load(f)["workloads"][WORKLOAD]
 
 
-def cpu_baseline(model, seconds=15.0, seconds_1thread=5.0):
-    """FP64 oracle (OpenMP, dSFMT streams seeded seed+thread) on a bounded
-    sample of the same workload; photons/s on min(16, host cores) threads,
-    and on one thread for a smaller sample."""
+def host_cores():
+    """Cores this process may run on (the OpenMP thread count of the CPU
+    baseline, like the reference's useAllCPUs, MCmatlab.c:727-728)."""
+    try:
+        return max(1, len(os.sched_getaffinity(0)))
+    except AttributeError:
+        return max(1, os.cpu_count() or 1)
+
+
+def cpu_model():
+    try:
+        with open("/proc/cpuinfo") as f:
+            for ln in f:
+                if ln.startswith("model name"):
+                    return ln.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown"
+
+
+def _oracle(fast=False):
     from mcmatlab_amd import abi
 
-    lib = C.CDLL(os.path.join(ROOT, "oracle", "_build", "liboracle_omp.so"))
+    lib = C.CDLL(os.path.join(ROOT, "oracle", "_build", "liboracle_fast.so" if fast else "liboracle_omp.so"))
     lib.oracle_mc_run.argtypes = [C.POINTER(abi.mcm_mc_input), C.POINTER(abi.mcm_mc_output), C.c_int, C.c_int, C.c_uint64, C.c_void_p]
-    cores = max(1, min(16, os.cpu_count() or 1))
+    lib.oracle_heat_run.argtypes = [C.POINTER(abi.mcm_heat_input), C.POINTER(abi.mcm_heat_output), C.c_int, C.c_int]
+    return lib
 
-    def run(n, threads):
+
+def _median(xs):
+    xs = sorted(xs)
+    k = len(xs)
+    return xs[k // 2] if k % 2 else 0.5 * (xs[k // 2 - 1] + xs[k // 2])
+
+
+def cpu_mc_rate(model, seconds=2.0, runs=5, threads=None, fast=False, want_absorbed=False):
+    """photons/s of the FP64 CPU restatement (dSFMT streams seeded seed +
+    thread) on `threads` cores: one warm-up run sizes the sample to about
+    `seconds`, then the median of `runs` runs, each timed like the
+    reference's simulationTime (the photon loop only)."""
+    import numpy as np
+
+    from mcmatlab_amd import abi
+
+    lib = _oracle(fast)
+    threads = threads or host_cores()
+
+    def run(n, seed):
         model.MC.nPhotonsRequested = float(n)
         inp = abi.MCInputs(model, seed=0)
         out = abi.MCOutputs(inp)
-        t = time.perf_counter()
-        rc = lib.oracle_mc_run(C.byref(inp.struct), C.byref(out.struct), threads, 0, 0, None)
-        dt = time.perf_counter() - t
+        rc = lib.oracle_mc_run(C.byref(inp.struct), C.byref(out.struct), threads, 0, seed, None)
         if rc != 0:
             raise RuntimeError("oracle failed")
-        return out.struct.nPhotons, dt
+        # the reference's simulationTime (minutes): the photon loop alone,
+        # not set-up or normalisation (MCmatlab.c:725, 742)
+        return out, out.struct.simulationTime * 60.0
 
-    n0, dt0 = run(1000 * cores, cores)
-    n = max(2000, int(n0 / max(dt0, 1e-3) * seconds))
-    n1, dt1 = run(n, cores)
-    n2, dt2 = run(max(500, int(n1 / dt1 / cores * seconds_1thread)), 1)
-    return {"value": n1 / dt1, "unit": "photons/s", "cores": cores, "kind": "port",
-            "sample": f"{int(n1)} packets of the {WORKLOAD} workload, FP64 transport, dSFMT, OpenMP ({dt1:.1f} s)",
-            "value_1thread": n2 / dt2, "sample_1thread": f"{int(n2)} packets, 1 thread ({dt2:.1f} s)"}
+    out, dt0 = run(500 * threads, 1)
+    out, dt0 = run(max(1000, int(out.struct.nPhotons / max(dt0, 1e-3) * 0.5)), 2)  # the second warm-up sizes the sample
+    n = max(1000, int(out.struct.nPhotons / max(dt0, 1e-3) * seconds))
+    rates, absorbed = [], []
+    mua = np.asarray(model.MC.mediaProperties["mua"])[:, 0]
+    for k in range(runs):
+        out, dt = run(n, 1000 * (k + 2))
+        rates.append(out.struct.nPhotons / dt)
+        if want_absorbed:
+            nfr = np.asarray(out.result()["NFR"], np.float64).reshape(model.MC.M.shape + (-1,), order="F")[..., 0]
+            absorbed.append((float(np.sum(nfr * mua[model.MC.M - 1]) * model.G.dx * model.G.dy * model.G.dz), int(out.struct.nPhotons)))
+    r = {"value": _median(rates), "runs": [round(x, 1) for x in rates], "packets_per_run": n, "threads": threads}
+    if want_absorbed:
+        r["absorbed"] = absorbed
+    return r
+
+
+def cpu_heat_rate(n=200, steps=(20, 8), runs=5, threads=None):
+    """voxel-steps/s of the CPU restatement of the heat propagator on the
+    200^3 workload: FP64 explicit (the benchmark's scheme) and the
+    reference's own FP32 Douglas-Gunn ADI, median of `runs` runs."""
+    from mcmatlab_amd import abi
+    from mcmatlab_amd.workloads import layered_tissue_heat
+
+    lib = _oracle()
+    threads = threads or host_cores()
+    T0, Om0, Pon, _ = layered_tissue_heat(n=n, steps_on=1, steps_off=1)
+    out = {}
+    for name, variant, k in (("fp64_explicit", 2, steps[0]), ("fp32_dgadi", 0, steps[1])):
+        P = dict(Pon, steps=k)
+        inp = abi.HeatInputs(T0, Om0, P)
+        rates = []
+        for _ in range(runs + 1):
+            o = abi.HeatOutputs(inp, want_f64=False)
+            t = time.perf_counter()
+            if lib.oracle_heat_run(C.byref(inp.struct), C.byref(o.struct), variant, threads) != 0:
+                raise RuntimeError("heat oracle failed")
+            rates.append(n ** 3 * k / (time.perf_counter() - t))
+        out[name] = {"voxel_steps_per_s": _median(rates[1:]), "steps_per_run": k}
+    return out
+
+
+def cpu_baseline(model):
+    """The reference algorithm's CPU path as a baseline beside the GPU: the
+    FP64 restatement (dSFMT, OpenMP over every core this process may use) on
+    a bounded sample of the same workload, -O3 and the shipped -Ofast flags
+    (MCmatlab.c:60); the larger rate is `value`.  Also one thread, and the
+    heat restatement on the same grid."""
+    o3 = cpu_mc_rate(model, want_absorbed=True)
+    fast = cpu_mc_rate(model, fast=True)
+    one = cpu_mc_rate(model, seconds=1.5, runs=3, threads=1)
+    best = max(o3["value"], fast["value"])
+    return {"value": best, "unit": "photons/s", "cores": o3["threads"], "kind": "port", "cpu": cpu_model(),
+            "sample": f"{o3['packets_per_run']} packets of the {WORKLOAD} workload per run, median of 5 runs after a warm-up; "
+                      "FP64 transport, dSFMT, OpenMP",
+            "o3": o3["value"], "ofast": fast["value"], "runs_o3": o3["runs"], "runs_ofast": fast["runs"],
+            "value_1thread": one["value"], "heat": cpu_heat_rate(), "_absorbed": o3["absorbed"]}
 
 
 def bench_heat(hip, steps_on=667, steps_off=1333, scheme="dgadi"):
@@ -220,6 +307,37 @@ def bench_spectral(seed=7):
             "absorbed_fraction_first_last": [absorbed[0], absorbed[-1]]}
 
 
+def bench_config1(packets=100_000_000):
+    """The CPU-runnable workload: 100^3 matched layered skin, pencil beam.
+    GPU photons/s (autotuned fluence copies, inputs resident) and the CPU
+    restatement on every core (median of 5 runs)."""
+    from mcmatlab_amd.api import MCPlan
+    from mcmatlab_amd.workloads import layered_tissue_mc
+
+    model = layered_tissue_mc(n=100, matched=True, beam="pencil", nPhotons=packets)
+    plan = MCPlan(model)
+    tune = plan.autotune()
+    best = min(t for t in tune.values() if t > 0)
+    copies = min(R for R, t in tune.items() if 0 < t <= 1.10 * best)
+    plan.set_fluence_copies(copies)
+    hip = Hip()
+    plan.simulate(1 << 20, seed=1)
+    plan.normalize()
+    hip.sync()
+    t = time.perf_counter()
+    done = 0
+    while done < packets:
+        n = min(LAUNCH_PACKETS, packets - done)
+        plan.simulate(n, seed=2, firstPhoton=done)
+        done += n
+    hip.sync()
+    dt = time.perf_counter() - t
+    plan.close()
+    cpu = cpu_mc_rate(layered_tissue_mc(n=100, matched=True, beam="pencil", nPhotons=1000), fast=True)
+    return {"workload": "layered_tissue_100_matched_pencil", "photons_per_s": packets / dt, "fluence_copies": copies,
+            "cpu_photons_per_s": cpu["value"], "cpu_cores": cpu["threads"], "cpu_flags": "-Ofast", "vs_cpu": packets / dt / cpu["value"]}
+
+
 def bench_heat_line(scheme="dgadi"):
     """The heat propagator alone: 667 lights-on + 1333 lights-off FP64 steps
     on the 200^3 grid, against the compulsory traffic of 41 / 33 bytes per
@@ -248,9 +366,18 @@ def bench_grad(n=128, packets=10_000_000, seed=3):
     t = time.perf_counter()
     f = plan.absorbed_power(box, packets, seed=seed, muaMap=mua, grad=False)
     dt_fwd = time.perf_counter() - t
+    # the production transport (fluence and escapes) on the same grid and
+    # medium; it reads the media table (mua 1/cm), not the per-voxel map
+    plan.simulate(100000, seed=seed + 2)
+    plan.normalize()
+    t = time.perf_counter()
+    plan.simulate(packets, seed=seed)
+    plan.normalize()
+    dt_tr = time.perf_counter() - t
     plan.close()
     return {"metric": "packets/s, absorbed power + dJ/dmua (128^3 blob absorber, 16^3 box)", "value": packets / dt,
-            "unit": "packets/s", "seconds": dt, "forward_s": dt_fwd, "grad_over_forward": dt / dt_fwd, "J": r["J"],
+            "unit": "packets/s", "seconds": dt, "forward_s": dt_fwd, "grad_over_forward": dt / dt_fwd,
+            "transport_forward_s": dt_tr, "grad_over_transport_forward": dt / dt_tr, "J": r["J"],
             "J_forward": f["J"], "J_rel_se": r["J_se"] / r["J"], "dJ_dlogmus": r["dJ_dlogmus"],
             "dJ_dlogmus_rel_se": abs(r["dJ_dlogmus_se"] / r["dJ_dlogmus"])}
 
@@ -409,6 +536,7 @@ def main():
     ap.add_argument("--packets", type=float, default=1e8)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-heat", action="store_true")
+    ap.add_argument("--no-extra", action="store_true", help="skip the batched-vs-plain, gradient-cost and 100^3 lines")
     ap.add_argument("--workload", default="layered_tissue_200_fresnel_tophat",
                     choices=["layered_tissue_200_fresnel_tophat", "spectral64", "heat200", "heat200explicit", "grad128", "adjoint128",
                              "sweep32", "binning200"],
@@ -519,9 +647,34 @@ def main():
             line["heat_explicit"] = bench_heat(hip, scheme="explicit")
         except Exception as e:  # heat kernels not available in this build
             line["heat"] = {"error": str(e)[:200]}
+    if not args.no_extra:
+        # the metric's other two parts: batched vs plain launches, and the
+        # cost of a forward + gradient against a forward
+        sp = bench_spectral()
+        line["batched_vs_plain"] = {"workload": "layered_tissue_150_spectral64", "packets": sp["packets"], "batched_s": sp["batched_s"],
+                                    "plain_launches_s": sp["plain_launches_s"], "plain_calls_s": sp["plain_calls_s"],
+                                    "speedup_vs_plain_launches": sp["speedup_vs_plain_launches"],
+                                    "speedup_vs_plain_calls": sp["speedup_vs_plain_calls"]}
+        gr = bench_grad()
+        line["grad_over_forward"] = {"workload": "blob_absorber_128", "packets": 10_000_000, "forward_plus_grad_s": gr["seconds"],
+                                     "forward_s": gr["forward_s"], "ratio": gr["grad_over_forward"],
+                                     "transport_forward_s": gr["transport_forward_s"],
+                                     "ratio_vs_transport": gr["grad_over_transport_forward"]}
     if not args.no_cpu:
-        line["cpu_baseline"] = cpu_baseline(layered_tissue_mc(n=200, matched=False, beam="tophat", nPhotons=1000))
-        line["vs_cpu"] = value / line["cpu_baseline"]["value"]
+        cb = cpu_baseline(layered_tissue_mc(n=200, matched=False, beam="tophat", nPhotons=1000))
+        # the GPU's absorbed fraction against the oracle's on the same
+        # workload: per packet the absorbed fraction lies in [0, 1], so its
+        # variance is at most p (1 - p)
+        ab = cb.pop("_absorbed")
+        n_cpu = sum(n for _, n in ab)
+        p_cpu = sum(a * n for a, n in ab) / n_cpu
+        se = math.sqrt(max(p_cpu * (1 - p_cpu), 1e-12) / n_cpu)
+        line["config"]["absorbed_fraction_oracle"] = p_cpu
+        line["config"]["absorbed_fraction_z"] = (absorbed - p_cpu) / se
+        line["cpu_baseline"] = cb
+        line["vs_cpu"] = value / cb["value"]
+        if not args.no_extra:
+            line["config1"] = bench_config1()
     print(json.dumps(line), flush=True)
 
 

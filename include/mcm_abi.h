@@ -293,6 +293,14 @@ int mcm_heat_plan_advance(mcm_heat_plan *plan, int64_t steps, double dt, int32_t
  * reference launches its kernels step by step,
  * finiteElementHeatPropagator.c:764-798). */
 int mcm_heat_plan_set_graph_steps(mcm_heat_plan *plan, int64_t steps);
+/* Explicit scheme only: z planes per workgroup of the stencil (8, 12, 16, 24
+ * or 32).  A plan picks it at creation by timing the candidates once per
+ * grid shape in the process (plans created for fewer than 64 steps take
+ * 16 unless the shape was timed before; environment MCM_HEAT_ZRUN forces
+ * one); the decomposition does not change any result.  No reference
+ * counterpart (launch shape of this implementation). */
+int mcm_heat_plan_set_explicit_zrun(mcm_heat_plan *plan, int32_t zPlanes);
+int32_t mcm_heat_plan_explicit_zrun(const mcm_heat_plan *plan);
 int mcm_heat_plan_download(mcm_heat_plan *plan, mcm_heat_output *out);
 
 /* Several plans (different grids and models, one device) advanced together:
@@ -317,6 +325,11 @@ int mcm_heat_plan_destroy(mcm_heat_plan *plan);
  *            has E != 0),
  *   dJ_dQ  = dJ / d dTdt_abs per voxel (lights-on steps only) [1 per K/s],
  *   dJ_dT0 = dJ / d T(initial) per voxel [1/K],
+ *   dJ_dc  = dJ / d dTdtperdeltaT [nM,nM,3] (the conduction coefficients
+ *            c[a,b,d] = TC_eff(a,b) / VHC(a) / d^2 of
+ *            simulateHeatDistribution.m:469-494, which the thermal
+ *            conductivities enter through; chain rule to TC on the caller's
+ *            side) [K s],
  * the exact derivatives of the discrete scheme (up to rounding).  States are
  * checkpointed every checkpointEvery steps (0: 100) and recomputed one
  * segment at a time on the way back.  Needs precision MCM_HEAT_FP64, scheme
@@ -332,6 +345,7 @@ typedef struct mcm_heat_adjoint_output {
   double *dJ_dQ;      /* caller-owned [nx,ny,nz] or NULL */
   double *dJ_dT0;     /* caller-owned [nx,ny,nz] or NULL */
   double *Omega_f64;  /* caller-owned final damage [nx,ny,nz] or NULL */
+  double *dJ_dc;      /* caller-owned [nM,nM,3] (index a + b nM + d nM^2) or NULL */
 } mcm_heat_adjoint_output;
 int mcm_heat_adjoint_run(const mcm_heat_input *in, const mcm_heat_segment *segments, int64_t nSegments,
                          const uint8_t *targetMask, int64_t checkpointEvery, mcm_heat_adjoint_output *out);

@@ -33,6 +33,8 @@ SIGNATURES = {
     "mcm_heat_plan_create": (C.c_int, [C.POINTER(abi.mcm_heat_input), C.POINTER(C.c_void_p)]),
     "mcm_heat_plan_advance": (C.c_int, [C.c_void_p, C.c_int64, C.c_double, C.c_int32]),
     "mcm_heat_plan_set_graph_steps": (C.c_int, [C.c_void_p, C.c_int64]),
+    "mcm_heat_plan_set_explicit_zrun": (C.c_int, [C.c_void_p, C.c_int32]),
+    "mcm_heat_plan_explicit_zrun": (C.c_int32, [C.c_void_p]),
     "mcm_heat_group_create": (C.c_int, [C.POINTER(C.c_void_p), C.c_int64, C.POINTER(C.c_void_p)]),
     "mcm_heat_group_advance": (C.c_int, [C.c_void_p, C.c_int64, C.POINTER(C.c_double), C.POINTER(C.c_int32), C.c_int64]),
     "mcm_heat_group_destroy": (C.c_int, [C.c_void_p]),

@@ -272,7 +272,7 @@ class mcm_heat_segment(C.Structure):
 
 class mcm_heat_adjoint_output(C.Structure):
     _fields_ = [("J", C.c_double), ("dJ_dQ", C.POINTER(C.c_double)), ("dJ_dT0", C.POINTER(C.c_double)),
-                ("Omega_f64", C.POINTER(C.c_double))]
+                ("Omega_f64", C.POINTER(C.c_double)), ("dJ_dc", C.POINTER(C.c_double))]
 
 
 class mcm_mc_item(C.Structure):

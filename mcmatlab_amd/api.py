@@ -145,6 +145,15 @@ class HeatPlan:
     def set_graph_steps(self, k):
         check(lib().mcm_heat_plan_set_graph_steps(self.h, int(k)))
 
+    def set_explicit_zrun(self, z):
+        """Explicit scheme: z planes per workgroup of the stencil (8, 12, 16,
+        24 or 32); results do not depend on it."""
+        check(lib().mcm_heat_plan_set_explicit_zrun(self.h, int(z)))
+
+    @property
+    def explicit_zrun(self):
+        return int(lib().mcm_heat_plan_explicit_zrun(self.h))
+
     def advance(self, steps, dt, lightsOn):
         check(lib().mcm_heat_plan_advance(self.h, int(steps), float(dt), int(bool(lightsOn))))
         self.steps += int(steps)
@@ -271,6 +280,13 @@ def fr_dependent_MCmatlab(model, medium, params_of_FR, nBins, nIterations, seed=
         m = model.copy()
         m.MC.M = Mf.reshape(M0.shape, order="F").astype(np.uint8)
         m.MC.mediaProperties = mp
+        # deposition-criteria medium bounds follow the split: a lower bound on
+        # the medium starts at its first sub-medium, an upper bound ends at
+        # its last, later media move up (getOpticalMediaProperties.m:126-133)
+        dc = m.MC.depositionCriteria
+        lo, hi = float(dc.minSubmediaIdx), float(dc.maxSubmediaIdx)
+        dc.minSubmediaIdx = lo + nBins - 1 if lo > medium else lo
+        dc.maxSubmediaIdx = hi + nBins - 1 if hi >= medium else hi
         m.MC.nPhotonsRequested = nPh * 0.5 ** (int(nIterations) - i)
         res = MCmatlab(m, 1, seed=None if seed is None else seed + i)
         if i < nIterations:
@@ -298,6 +314,10 @@ def t_dependent_heat(T, Omega, P, M, VHC, TC, d, medium, thermal_of_T, nBins, up
 
     M = np.asarray(M)
     nM = len(VHC)
+    if not 1 <= medium <= nM:
+        raise ValueError("medium out of range")
+    if nM - 1 + nBins > 255:
+        raise ValueError("more than 255 media after splitting")
     vox = (M == medium).ravel(order="F")
     if not vox.any():
         raise ValueError("the medium has no voxels")
@@ -332,13 +352,16 @@ def t_dependent_heat(T, Omega, P, M, VHC, TC, d, medium, thermal_of_T, nBins, up
     return state[0], state[1], Ms
 
 
-def heat_adjoint(T, Omega, P, segments=None, targetMask=None, checkpointEvery=0, grads=True, T_f64=None, Omega_f64=None):
+def heat_adjoint(T, Omega, P, segments=None, targetMask=None, checkpointEvery=0, grads=True, T_f64=None, Omega_f64=None,
+                 coefGrad=False):
     """mcm_heat_adjoint_run: final damage J summed over the target voxels
     (targetMask, default: media with E != 0) after the schedule `segments`
     = [(steps, dt, lightsOn), ...] (default: P's steps, dt, lightsOn) of FP64
     DG-ADI steps, and its exact discrete gradients with respect to the
-    heating rate dTdt_abs and the initial temperature.  Returns a dict J,
-    dJ_dQ, dJ_dT0 ([nx,ny,nz] Fortran order, or None), Omega (final FP64
+    heating rate dTdt_abs and the initial temperature (grads) and to the
+    conduction coefficients dTdtperdeltaT (coefGrad; media.conductivity_gradient
+    turns it into dJ/dTC).  Returns a dict J, dJ_dQ, dJ_dT0 ([nx,ny,nz]
+    Fortran order, or None), dJ_dc ([nM,nM,3] or None), Omega (final FP64
     damage)."""
     inp = abi.HeatInputs(T, Omega, P, precision=abi.MCM_HEAT_FP64, scheme=abi.MCM_HEAT_DGADI, T_f64=T_f64, Omega_f64=Omega_f64)
     shape = (int(inp.struct.nx), int(inp.struct.ny), int(inp.struct.nz))
@@ -358,11 +381,63 @@ def heat_adjoint(T, Omega, P, segments=None, targetMask=None, checkpointEvery=0,
         gq, gt = np.zeros(L), np.zeros(L)
         out.dJ_dQ = gq.ctypes.data_as(C.POINTER(C.c_double))
         out.dJ_dT0 = gt.ctypes.data_as(C.POINTER(C.c_double))
+    gc = None
+    if coefGrad:
+        nM = int(inp.struct.nM)
+        gc = np.zeros(nM * nM * 3)
+        out.dJ_dc = gc.ctypes.data_as(C.POINTER(C.c_double))
     check(lib().mcm_heat_adjoint_run(C.byref(inp.struct), arr, len(segs),
                                      None if mask is None else mask.ctypes.data_as(C.POINTER(C.c_uint8)),
                                      int(checkpointEvery), C.byref(out)))
     f = lambda a: None if a is None else a.reshape(shape, order="F")
-    return {"J": out.J, "dJ_dQ": f(gq), "dJ_dT0": f(gt), "Omega": f(om)}
+    return {"J": out.J, "dJ_dQ": f(gq), "dJ_dT0": f(gt), "Omega": f(om),
+            "dJ_dc": None if gc is None else gc.reshape(int(inp.struct.nM), int(inp.struct.nM), 3, order="F")}
+
+
+def heat_damage_function():
+    """A torch.autograd.Function over mcm_heat_adjoint_run: forward(Q, T0, TC,
+    setup) returns the final damage J (a 0-d float64 tensor) of the FP64
+    DG-ADI schedule; backward gives dJ/dQ (heating rate dTdt_abs [nx,ny,nz]),
+    dJ/dT0 (initial temperature [nx,ny,nz]) and dJ/dTC (thermal conductivity
+    per medium [nM]) times the incoming gradient.  `setup` is a dict with the
+    heat MEX fields P (M, A, E, sensors, heatBoundaryType; its dTdt_abs and
+    dTdtperdeltaT are replaced), VHC [nM], d = (dx, dy, dz), segments
+    [(steps, dt, lightsOn), ...] and optionally targetMask and
+    checkpointEvery.  Usage:
+        HD = heat_damage_function()
+        J = HD.apply(Q, T0, TC, setup)
+        J.backward()   # Q.grad, T0.grad, TC.grad
+    """
+    import torch
+
+    from .media import conductivity_gradient, heat_coefficients
+
+    class HeatDamage(torch.autograd.Function):
+        @staticmethod
+        def forward(ctx, Q, T0, TC, setup):
+            VHC = np.asarray(setup["VHC"], np.float64)
+            dx, dy, dz = setup["d"]
+            tc = TC.detach().cpu().numpy().astype(np.float64)
+            P = dict(setup["P"])
+            P["dTdt_abs"] = np.asfortranarray(Q.detach().cpu().numpy().astype(np.float32))
+            P["dTdtperdeltaT"] = heat_coefficients(VHC, tc, dx, dy, dz)
+            t0 = np.asfortranarray(T0.detach().cpu().numpy().astype(np.float64))
+            om0 = np.zeros(t0.shape, np.float32)
+            r = heat_adjoint(t0.astype(np.float32), om0, P, segments=setup["segments"], targetMask=setup.get("targetMask"),
+                             checkpointEvery=setup.get("checkpointEvery", 0), T_f64=t0, coefGrad=True)
+            gtc = conductivity_gradient(r["dJ_dc"], VHC, tc, dx, dy, dz)
+            dev = Q.device
+            ctx.save_for_backward(torch.from_numpy(np.ascontiguousarray(r["dJ_dQ"])).to(dev, Q.dtype),
+                                  torch.from_numpy(np.ascontiguousarray(r["dJ_dT0"])).to(dev, T0.dtype),
+                                  torch.from_numpy(gtc).to(dev, TC.dtype))
+            return torch.tensor(r["J"], dtype=torch.float64, device=dev)
+
+        @staticmethod
+        def backward(ctx, gJ):
+            gq, gt, gtc = ctx.saved_tensors
+            return gJ.to(gq.dtype) * gq, gJ.to(gt.dtype) * gt, gJ.to(gtc.dtype) * gtc, None
+
+    return HeatDamage
 
 
 def heat_adjoints(cases, workers=4, **kw):

@@ -28,6 +28,10 @@
 // stencils) follows the forward kernels.
 #include <hip/hip_runtime.h>
 
+#include <array>
+#include <map>
+#include <mutex>
+
 #include <algorithm>
 #include <cmath>
 #include <cstdlib>
@@ -1334,6 +1338,114 @@ __global__ void __launch_bounds__(kHeatTPB) heat_adj_combine(HeatArgs<double> a,
   if (lightsOn) dQ[i] += dt * wv(mu1, x, y, z);
 }
 
+// Gradient with respect to the conduction coefficients c[a, b, d] (the
+// reference's dTdtperdeltaT).  Perturbing c in the step's equations and
+// applying the adjoints mu1..mu3 gives, with dL_d v at i = sum over the
+// d-neighbours n of dc[M_i, M_n, d] (v_n - v_i),
+//   dJ = dt/2 mu1^T D dL_x (u + T1)
+//      + mu1^T D dL_y (dt u) + mu2^T D dL_y (dt/2 (T2 - u))
+//      + mu1^T D dL_z (dt u) + mu3^T D dL_z (dt/2 (T3 - u)).
+// The intermediate states follow from the step's two ends by undoing the
+// line solves: T2 = A_z T3 + dt/2 D L_z u = T3 - dt/2 D L_z (T3 - u) and
+// T1 = T2 - dt/2 D L_y (T2 - u).
+
+// lo = hi - dt/2 D L_AXIS (hi - u)
+template <int AXIS>
+__global__ void __launch_bounds__(kHeatTPB) heat_adj_undo(HeatArgs<double> a, const double* __restrict__ hi, const double* __restrict__ u,
+                                                         double* __restrict__ lo) {
+  const int nx = a.nx, ny = a.ny, nz = a.nz, nxy = nx * ny;
+  const int p = blockIdx.x * blockDim.x + threadIdx.x, z = blockIdx.y;
+  if (p >= nxy) return;
+  const int x = p % nx, y = p / nx;
+  const int64_t i = (int64_t)z * nxy + p;
+  const double vi = hi[i] - u[i];
+  double l = 0;
+  if (!heat_frozen(a, x, y, z)) {
+    const int q = AXIS == 1 ? y : z, n = AXIS == 1 ? ny : nz;
+    const int64_t st = AXIS == 1 ? nx : nxy;
+    const int m = a.M[i];
+    if (q > 0) l += coef(a.c, a.nM, m, (int)a.M[i - st], AXIS) * ((hi[i - st] - u[i - st]) - vi);
+    if (q < n - 1) l += coef(a.c, a.nM, m, (int)a.M[i + st], AXIS) * ((hi[i + st] - u[i + st]) - vi);
+  }
+  lo[i] = hi[i] - a.dt / 2 * l;
+}
+
+// One step's contributions to dJ/dc, summed per (a, b, d) in LDS (one FP64
+// LDS atomic per wavefront where all its lanes hit the same coefficient,
+// the usual case away from interfaces) and added to the global sums; a
+// grid-stride loop keeps the number of global atomics per step small.
+constexpr int kCoefBins = 192;  // LDS bins: tables of up to 8 media
+template <bool LDS_BINS>
+__global__ void __launch_bounds__(kHeatTPB) heat_adj_coef(HeatArgs<double> a, const double* __restrict__ mu1, const double* __restrict__ mu2,
+                                                         const double* __restrict__ mu3, const double* __restrict__ u,
+                                                         const double* __restrict__ T1, const double* __restrict__ T2,
+                                                         const double* __restrict__ T3, double* dJdc) {
+  __shared__ double sB[LDS_BINS ? kCoefBins : 1];
+  const int nM = a.nM, nB = nM * nM * 3;
+  if (LDS_BINS) {
+    for (int k = threadIdx.x; k < nB; k += blockDim.x) sB[k] = 0;
+    __syncthreads();
+  }
+  const int nx = a.nx, ny = a.ny, nz = a.nz, nxy = nx * ny;
+  const int64_t L = (int64_t)nxy * nz;
+  const double dt = a.dt;
+  const int lane = threadIdx.x & 63;
+  const int64_t span = (int64_t)gridDim.x * blockDim.x;
+  // whole wavefronts run the loop together (shuffles below need every lane)
+  const int64_t iters = (L + span - 1) / span;
+  for (int64_t it = 0; it < iters; ++it) {
+    const int64_t i = it * span + (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const bool live = i < L;
+    int x = 0, y = 0, z = 0, m = 0;
+    bool act = false;
+    if (live) {
+      z = (int)(i / nxy);
+      const int p = (int)(i - (int64_t)z * nxy);
+      x = p % nx, y = p / nx;
+      m = a.M[i];
+      act = !heat_frozen(a, x, y, z);
+    }
+    for (int d = 0; d < 3; ++d) {
+      for (int s = -1; s <= 1; s += 2) {
+        double g = 0;
+        int k = -1;
+        if (act) {
+          const int q = (d == 0 ? x : d == 1 ? y : z) + s, n = d == 0 ? nx : d == 1 ? ny : nz;
+          if (q >= 0 && q < n) {
+            const int64_t st = d == 0 ? 1 : d == 1 ? nx : nxy;
+            const int64_t j = i + s * st;
+            const int mn = a.M[j];
+            k = m + mn * nM + d * nM * nM;
+            if (d == 0) {
+              g = dt / 2 * mu1[i] * ((u[j] + T1[j]) - (u[i] + T1[i]));
+            } else {
+              const double* W = d == 1 ? mu2 : mu3;
+              const double* V = d == 1 ? T2 : T3;
+              g = dt * mu1[i] * (u[j] - u[i]) + dt / 2 * W[i] * ((V[j] - u[j]) - (V[i] - u[i]));
+            }
+          }
+        }
+        if (LDS_BINS) {
+          const int k0 = __shfl(k, 0);
+          if (__all(k == k0) && k0 >= 0) {
+            for (int off = 32; off > 0; off >>= 1) g += __shfl_xor(g, off);
+            if (lane == 0) atomicAdd(&sB[k0], g);
+          } else if (k >= 0) {
+            atomicAdd(&sB[k], g);
+          }
+        } else if (k >= 0 && g != 0) {
+          atomicAdd(&dJdc[k], g);
+        }
+      }
+    }
+  }
+  if (LDS_BINS) {
+    __syncthreads();
+    for (int k = threadIdx.x; k < nB; k += blockDim.x)
+      if (sB[k] != 0) atomicAdd(&dJdc[k], sB[k]);
+  }
+}
+
 }  // namespace mcm
 
 using namespace mcm;
@@ -1461,8 +1573,39 @@ HeatArgs<RT> make_args(mcm_heat_plan* P, double dt, int lightsOn) {
 // other state buffer (overwritten by the first real step), a scratch damage
 // field (the DG-ADI work buffer, unused by this scheme) and scratch maxima;
 // no sensor is recorded.
+// The choice is kept per grid shape, precision and device for the process,
+// so later plans of the same shape (a schedule cut into many calls) do not
+// time again; a plan created for fewer than kTuneMinSteps steps does not
+// time at all and takes kDefaultZRun, unless the shape is known.
+constexpr int kDefaultZRun = 16;
+constexpr int64_t kTuneMinSteps = 64;
+std::mutex g_zrunMutex;
+std::map<std::array<int, 6>, int> g_zrunCache;
+
+bool valid_zrun(int zr) { return zr == 8 || zr == 12 || zr == 16 || zr == 24 || zr == 32; }
+
 template <typename RT>
-int tune_explicit(mcm_heat_plan* P) {
+int tune_explicit(mcm_heat_plan* P, int64_t steps) {
+  const std::array<int, 6> key = {P->device, P->nx, P->ny, P->nz, P->fp32, P->nM * P->nM * 3 <= kLdsCoef};
+  if (const char* e = std::getenv("MCM_HEAT_ZRUN")) {
+    const int zr = std::atoi(e);
+    if (valid_zrun(zr)) {
+      P->explicitZRun = zr;
+      return MCM_OK;
+    }
+  }
+  {
+    std::lock_guard<std::mutex> lk(g_zrunMutex);
+    auto it = g_zrunCache.find(key);
+    if (it != g_zrunCache.end()) {
+      P->explicitZRun = it->second;
+      return MCM_OK;
+    }
+  }
+  if (steps < kTuneMinSteps) {
+    P->explicitZRun = kDefaultZRun;
+    return MCM_OK;
+  }
   HeatArgs<RT> a = make_args<RT>(P, 1e-9, 1);
   a.Om = (RT*)P->B.p;
   HeatBuf keys;
@@ -1488,6 +1631,8 @@ int tune_explicit(mcm_heat_plan* P) {
   MCM_HIP(hipGetLastError());
   MCM_HIP(hipStreamSynchronize(P->stream));
   P->explicitZRun = bestZ;
+  std::lock_guard<std::mutex> lk(g_zrunMutex);
+  g_zrunCache[key] = bestZ;
   return MCM_OK;
 }
 
@@ -1638,6 +1783,9 @@ int step_graph(mcm_heat_plan* P, int64_t K, double dt, int lightsOn, hipGraphExe
   (void)hipGraphDestroy(graph);
   if (instErr != hipSuccess) return fail(MCM_ERR_DEVICE, "instantiating the heat step graph: %s", hipGetErrorString(instErr));
   if (P->graphs.size() >= 4) {
+    // the oldest graph may still be queued or running on the stream: an
+    // executable graph is destroyed only once its launches have finished
+    MCM_HIP(hipStreamSynchronize(P->stream));
     (void)hipGraphExecDestroy(P->graphs.front().exec);
     P->graphs.erase(P->graphs.begin());
   }
@@ -1889,7 +2037,7 @@ int mcm_heat_plan_create(const mcm_heat_input* in, mcm_heat_plan** out) {
   rc = P->fp32 ? upload_state<float>(P.get(), in) : upload_state<double>(P.get(), in);
   if (rc) return rc;
   if ((rc = ensure_sensor_capacity(P.get(), std::max<int64_t>(in->steps, 1) + 1))) return rc;
-  if (P->explicitScheme && (rc = P->fp32 ? tune_explicit<float>(P.get()) : tune_explicit<double>(P.get()))) return rc;
+  if (P->explicitScheme && (rc = P->fp32 ? tune_explicit<float>(P.get(), in->steps) : tune_explicit<double>(P.get(), in->steps))) return rc;
   MCM_HIP(hipStreamSynchronize(s));
   *out = P.release();
   return MCM_OK;
@@ -1968,6 +2116,9 @@ int mcm_heat_group_advance(mcm_heat_group* G, int64_t steps, const double* dt, c
       (void)hipGraphDestroy(graph);
       if (err != hipSuccess) return fail(MCM_ERR_DEVICE, "instantiating the group's step graph: %s", hipGetErrorString(err));
       if (G->graphs.size() >= 4) {
+        // replays of the oldest graph end on P0's stream (every branch joins
+        // it there): wait for them before the executable graph is destroyed
+        MCM_HIP(hipStreamSynchronize(P0->stream));
         (void)hipGraphExecDestroy(G->graphs.front().exec);
         G->graphs.erase(G->graphs.begin());
       }
@@ -2024,6 +2175,19 @@ int mcm_heat_group_destroy(mcm_heat_group* G) {
   delete G;
   return MCM_OK;
 }
+
+int mcm_heat_plan_set_explicit_zrun(mcm_heat_plan* P, int32_t zPlanes) {
+  if (!P) return fail(MCM_ERR_INVALID_ARGUMENT, "null plan");
+  if (!P->explicitScheme) return fail(MCM_ERR_INVALID_ARGUMENT, "the plan does not use the explicit scheme");
+  if (!valid_zrun(zPlanes)) return fail(MCM_ERR_INVALID_ARGUMENT, "z planes per workgroup must be 8, 12, 16, 24 or 32");
+  MCM_HIP(hipSetDevice(P->device));
+  MCM_HIP(hipStreamSynchronize(P->stream));
+  P->drop_graphs();  // captured steps hold the old launch shape
+  P->explicitZRun = zPlanes;
+  return MCM_OK;
+}
+
+int32_t mcm_heat_plan_explicit_zrun(const mcm_heat_plan* P) { return P ? P->explicitZRun : 0; }
 
 int mcm_heat_plan_set_graph_steps(mcm_heat_plan* P, int64_t steps) {
   if (!P) return fail(MCM_ERR_INVALID_ARGUMENT, "null plan");
@@ -2100,13 +2264,19 @@ int mcm_heat_adjoint_run(const mcm_heat_input* in, const mcm_heat_segment* segme
     if (mask[i]) J += om[i];
   out->J = J;
   if (out->Omega_f64) std::memcpy(out->Omega_f64, om.data(), bytes);
-  if (!out->dJ_dQ && !out->dJ_dT0) return MCM_OK;
+  if (!out->dJ_dQ && !out->dJ_dT0 && !out->dJ_dc) return MCM_OK;
   if (P->nz > 65535) return fail(MCM_ERR_UNSUPPORTED, "heat adjoint: nz > 65535 z planes");
   // reverse sweep, one checkpoint segment at a time
-  HeatBuf seg, lam, lamN, mu1, mu2, mu3, dQ;
+  HeatBuf seg, lam, lamN, mu1, mu2, mu3, dQ, s1, s2, dC;
+  const int nBins = P->nM * P->nM * 3;
   if ((rc = halloc(seg, (size_t)(C + 1) * bytes)) || (rc = halloc(lam, bytes)) || (rc = halloc(lamN, bytes)) || (rc = halloc(mu1, bytes)) ||
       (rc = halloc(mu2, bytes)) || (rc = halloc(mu3, bytes)) || (rc = halloc(dQ, bytes)))
     return rc;
+  if (out->dJ_dc) {
+    // the step's intermediate states and the coefficient sums
+    if ((rc = halloc(s1, bytes)) || (rc = halloc(s2, bytes)) || (rc = halloc(dC, (size_t)nBins * sizeof(double)))) return rc;
+    MCM_HIP(hipMemsetAsync(dC.p, 0, (size_t)nBins * sizeof(double), s));
+  }
   MCM_HIP(hipMemsetAsync(lam.p, 0, bytes, s));
   MCM_HIP(hipMemsetAsync(dQ.p, 0, bytes, s));
   auto segp = [&](int64_t k) { return (double*)seg.p + (size_t)k * L; };
@@ -2140,9 +2310,23 @@ int mcm_heat_adjoint_run(const mcm_heat_input* in, const mcm_heat_segment* segme
         hipLaunchKernelGGL(heat_adj_combine<false>, gc, dim3(tpb), 0, s, a, (const double*)mu1.p, (const double*)mu2.p,
                            (const double*)mu3.p, x0, x1, (const uint8_t*)dMask.p, (double*)lamN.p, (double*)dQ.p, lights[(size_t)n]);
       MCM_HIP(hipGetLastError());
+      if (out->dJ_dc) {
+        const dim3 gp((unsigned)((P->nx * P->ny + tpb - 1) / tpb), (unsigned)P->nz);
+        hipLaunchKernelGGL(heat_adj_undo<2>, gp, dim3(tpb), 0, s, a, x1, x0, (double*)s2.p);
+        hipLaunchKernelGGL(heat_adj_undo<1>, gp, dim3(tpb), 0, s, a, (const double*)s2.p, x0, (double*)s1.p);
+        const unsigned gcoef = (unsigned)std::min<int64_t>((L + tpb - 1) / tpb, 2048);
+        if (nBins <= kCoefBins)
+          hipLaunchKernelGGL(heat_adj_coef<true>, dim3(gcoef), dim3(tpb), 0, s, a, (const double*)mu1.p, (const double*)mu2.p,
+                             (const double*)mu3.p, x0, (const double*)s1.p, (const double*)s2.p, x1, (double*)dC.p);
+        else
+          hipLaunchKernelGGL(heat_adj_coef<false>, dim3(gcoef), dim3(tpb), 0, s, a, (const double*)mu1.p, (const double*)mu2.p,
+                             (const double*)mu3.p, x0, (const double*)s1.p, (const double*)s2.p, x1, (double*)dC.p);
+        MCM_HIP(hipGetLastError());
+      }
       std::swap(lam.p, lamN.p);
     }
   }
+  if (out->dJ_dc) MCM_HIP(hipMemcpyAsync(out->dJ_dc, dC.p, (size_t)nBins * sizeof(double), hipMemcpyDeviceToHost, s));
   if (out->dJ_dQ) MCM_HIP(hipMemcpyAsync(out->dJ_dQ, dQ.p, bytes, hipMemcpyDeviceToHost, s));
   if (out->dJ_dT0) MCM_HIP(hipMemcpyAsync(out->dJ_dT0, lam.p, bytes, hipMemcpyDeviceToHost, s));
   MCM_HIP(hipStreamSynchronize(s));

@@ -8,7 +8,8 @@ produces in MATLAB before calling the MEX files.
   central differences; the normal points into the voxel's own class, as in the
   reference, and is stored as the same spherical angles.
 - heat_coefficients / heat_dtmax: dTdtperdeltaT and the explicit stability
-  limit (simulateHeatDistribution.m:469-514).
+  limit (simulateHeatDistribution.m:469-514); conductivity_gradient: the
+  chain rule from dJ/d dTdtperdeltaT to dJ/dTC.
 - sensor_weights: tempSensorCornerIdxs / tempSensorInterpWeights
   (simulateHeatDistribution.m:543-609).
 - bin_medium: the sub-media of an FR/T/FD-dependent medium
@@ -51,6 +52,22 @@ def heat_coefficients(VHC, TC, dx, dy, dz):
     TCe[np.isnan(TCe)] = 0
     base = TCe / VHC[:, None]
     return np.stack([base / np.float32(dx ** 2), base / np.float32(dy ** 2), base / np.float32(dz ** 2)], axis=2).astype(np.float32)
+
+
+def conductivity_gradient(dJ_dc, VHC, TC, dx, dy, dz):
+    """dJ/dTC [nM] from dJ/dc [nM, nM, 3] (the heat adjoint's gradient with
+    respect to dTdtperdeltaT), by the chain rule through
+    c[a, b, d] = TC_eff(a, b) / VHC(a) / d^2, TC_eff = 2 TC_a TC_b / (TC_a + TC_b)
+    (simulateHeatDistribution.m:469-483): dTC_eff/dTC_a = 2 TC_b^2 / (TC_a + TC_b)^2."""
+    g = np.asarray(dJ_dc, dtype=np.float64)
+    VHC = np.asarray(VHC, dtype=np.float64)
+    TC = np.asarray(TC, dtype=np.float64)
+    w = (g[:, :, 0] / dx ** 2 + g[:, :, 1] / dy ** 2 + g[:, :, 2] / dz ** 2) / VHC[:, None]  # dJ / dTC_eff[a, b]
+    s = TC[:, None] + TC[None, :]
+    with np.errstate(invalid="ignore", divide="ignore"):
+        da = np.where(s > 0, 2 * TC[None, :] ** 2 / s ** 2, 0.0)  # dTC_eff[a, b] / dTC_a
+        db = np.where(s > 0, 2 * TC[:, None] ** 2 / s ** 2, 0.0)  # dTC_eff[a, b] / dTC_b
+    return (w * da).sum(axis=1) + (w * db).sum(axis=0)
 
 
 def heat_dtmax(M, VHC, TC, dx, dy, dz):

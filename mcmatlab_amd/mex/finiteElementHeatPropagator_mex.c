@@ -9,8 +9,8 @@
  * with P holding M (0-based uint8), A, E, dTdtperdeltaT, dTdt_abs, useAllCPUs,
  * heatBoundaryType, tempSensorCornerIdxs, tempSensorInterpWeights, lightsOn,
  * steps and dt.  Outputs have the reference's classes and sizes.  The device
- * state is FP64 (an optional P.precision = 'single' selects the reference's
- * FP32 arithmetic).
+ * state is FP64; an optional field P.precision = 'single' selects the
+ * reference's FP32 arithmetic ('double', the default, FP64).
  *
  * Build (MATLAB):   mex -R2018a finiteElementHeatPropagator_mex.c -I<repo>/include -L<repo>/mcmatlab_amd -lmcm
  *                       -output finiteElementHeatPropagator
@@ -52,6 +52,13 @@ void mexFunction(int nlhs, mxArray *plhs[], int nrhs, const mxArray *prhs[]) {
   in.tempSensorInterpWeights = mxGetPr(mxGetField(P, 0, "tempSensorInterpWeights"));
   in.useAllCPUs = mxIsLogicalScalarTrue(mxGetField(P, 0, "useAllCPUs"));
   in.precision = MCM_HEAT_FP64;
+  if (mxGetFieldNumber(P, "precision") >= 0) {  /* optional: 'single' or 'double' */
+    const mxArray *pr = mxGetField(P, 0, "precision");
+    char buf[8];
+    if (!mxIsChar(pr) || mxGetString(pr, buf, sizeof buf) != 0 || (strcmp(buf, "single") != 0 && strcmp(buf, "double") != 0))
+      mexErrMsgIdAndTxt("MCmatlab:invalidInput", "P.precision must be 'single' or 'double'");
+    if (strcmp(buf, "single") == 0) in.precision = MCM_HEAT_FP32;
+  }
   in.scheme = MCM_HEAT_DGADI;
   in.isInterruptPending = interrupt_poll;
 

@@ -18,7 +18,7 @@ mxINT32 = 12
 mxOBJECT = 18
 
 _CLASS_DTYPE = {mxDOUBLE: np.float64, mxSINGLE: np.float32, mxUINT8: np.uint8, mxLOGICAL: np.uint8,
-                mxINT32: np.int32, mxINT8: np.int8}
+                mxINT32: np.int32, mxINT8: np.int8, mxCHAR: np.uint16}
 
 
 class MexLib:
@@ -192,6 +192,8 @@ def run_heat_mex(m: MexLib, T, Omega, P, interrupt_after=-1):
         "tempSensorInterpWeights": m.num(np.asarray(P.get("tempSensorInterpWeights", np.zeros((0, 3))), dtype=np.float64).reshape(-1, 3)),
         "lightsOn": m.logical(P["lightsOn"]), "steps": m.num(P["steps"]), "dt": m.num(P["dt"]),
     }
+    if "precision" in P:  # optional: 'single' selects the reference's FP32 arithmetic
+        fields["precision"] = m.num(np.frombuffer(P["precision"].encode(), np.uint8).astype(np.uint16), mxCHAR)
     Ps = m.obj(fields, struct=True)
     Tm = m.num(T, mxSINGLE, shape=T.shape)
     Omm = m.num(Om, mxSINGLE, shape=Om.shape if Om.ndim >= 2 else None)

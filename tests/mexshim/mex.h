@@ -66,6 +66,9 @@ mxArray *mxGetPropertyShared(const mxArray *pa, mwIndex i, const char *name);
 void mxSetProperty(mxArray *pa, mwIndex i, const char *name, const mxArray *value);
 void mxSetPropertyShared(mxArray *pa, mwIndex i, const char *name, const mxArray *value);
 mxArray *mxGetField(const mxArray *pa, mwIndex i, const char *name);
+int mxGetFieldNumber(const mxArray *pa, const char *name);
+bool mxIsChar(const mxArray *pa);
+int mxGetString(const mxArray *pa, char *buf, mwSize buflen);
 double *mxGetPr(const mxArray *pa);
 void *mxGetData(const mxArray *pa);
 const mwSize *mxGetDimensions(const mxArray *pa);

@@ -92,6 +92,21 @@ mxArray *mxGetPropertyShared(const mxArray *pa, mwIndex i, const char *name) {
  * shared array is returned (the reference never frees these copies). */
 mxArray *mxGetProperty(const mxArray *pa, mwIndex i, const char *name) { return mxGetPropertyShared(pa, i, name); }
 mxArray *mxGetField(const mxArray *pa, mwIndex i, const char *name) { return mxGetPropertyShared(pa, i, name); }
+int mxGetFieldNumber(const mxArray *pa, const char *name) {
+  for (int k = 0; pa && k < pa->nfields; k++)
+    if (strcmp(pa->names[k], name) == 0) return k;
+  return -1;
+}
+bool mxIsChar(const mxArray *pa) { return pa && pa->cls == mxCHAR_CLASS; }
+/* mxChar (16-bit) to a NUL-terminated char string; 1 if it does not fit */
+int mxGetString(const mxArray *pa, char *buf, mwSize buflen) {
+  if (!pa || pa->cls != mxCHAR_CLASS || buflen == 0) return 1;
+  const uint16_t *c = (const uint16_t *)pa->data;
+  size_t k = 0;
+  for (; k < pa->nel && k + 1 < buflen; k++) buf[k] = (char)c[k];
+  buf[k] = 0;
+  return k < pa->nel ? 1 : 0;
+}
 void mxSetPropertyShared(mxArray *pa, mwIndex i, const char *name, const mxArray *value) {
   (void)i;
   set_field(pa, name, (mxArray *)value);

@@ -215,54 +215,61 @@ def blob128(api):
 
 
 BLOB_N = 10_000_000
-BLOB_SEEDS = (31, 32, 33, 34)
+BLOB_SEEDS = (31, 32, 33, 34, 35, 36)
 H_REL = 1e-3
 
 
 def test_blob128_gradients_match_central_differences(blob128):
     """dJ/dmua of single voxels and along a direction against
-    (J(mua + h) - J(mua - h)) / 2h with h = 1e-3 mua, the same packet streams
-    on both sides; both sides averaged over four seeds of 1e7 packets."""
+    (J(mua + h) - J(mua - h)) / 2h with h = 1e-3 mua and the same packet
+    streams on both sides (common random numbers), 1e7 packets per seed.
+
+    With common random numbers the difference quotient and the pathwise
+    gradient see the same packets and differ only where a packet's roulette
+    decision flips between mua + h and mua - h, so the per-seed differences
+    measure that noise: the three voxels J is most sensitive to, and the
+    direction of the map itself, agree within 2 % on the mean over the
+    seeds; the most sensitive voxel outside the target box (ten times
+    weaker, only the path-length term) agrees within 2 % or within four
+    standard errors of the per-seed differences, whichever is larger."""
     plan, mua, box = blob128
     runs = [plan.absorbed_power(box, BLOB_N, seed=s, muaMap=mua) for s in BLOB_SEEDS]
-    g = np.mean([r["dJ_dmua"] for r in runs], axis=0)
-    J = np.mean([r["J"] for r in runs])
-    assert J > 0 and np.isfinite(g).all()
+    gs = np.array([r["dJ_dmua"] for r in runs])
+    g = gs.mean(axis=0)
+    assert all(r["J"] > 0 for r in runs) and np.isfinite(g).all()
 
     def fd(direction, h):
-        Jp = np.mean([plan.absorbed_power(box, BLOB_N, seed=s, muaMap=mua + h * direction, grad=False)["J"] for s in BLOB_SEEDS])
-        Jm = np.mean([plan.absorbed_power(box, BLOB_N, seed=s, muaMap=mua - h * direction, grad=False)["J"] for s in BLOB_SEEDS])
-        return (Jp - Jm) / (2 * h)
+        out = []
+        for s in BLOB_SEEDS:
+            Jp = plan.absorbed_power(box, BLOB_N, seed=s, muaMap=mua + h * direction, grad=False)["J"]
+            Jm = plan.absorbed_power(box, BLOB_N, seed=s, muaMap=mua - h * direction, grad=False)["J"]
+            out.append((Jp - Jm) / (2 * h))
+        return np.array(out)
 
-    # the three voxels whose mua changes J the most (relative sensitivity),
-    # one inside the target box, the strongest outside it, the strongest overall
+    def check(name, per_seed_grad, per_seed_fd, allow_se):
+        diff = per_seed_fd - per_seed_grad
+        gm = float(per_seed_grad.mean())
+        se = float(diff.std(ddof=1) / math.sqrt(len(diff)))
+        rel = float(diff.mean()) / gm
+        print(f"[gpu-config] blob128 {name}: grad {gm:.6e} fd {per_seed_fd.mean():.6e} ({rel:+.2%}, se {se / abs(gm):.2%})", flush=True)
+        assert abs(diff.mean()) <= max(0.02 * abs(gm), 4 * se if allow_se else 0.0), (name, rel, se / abs(gm))
+
     sens = np.abs(g * mua)
     (x0, x1), (y0, y1), (z0, z1) = box
     inbox = np.zeros(mua.shape, bool)
     inbox[x0:x1, y0:y1, z0:z1] = True
-    picks = [np.unravel_index(np.argmax(np.where(inbox, sens, 0)), g.shape),
-             np.unravel_index(np.argmax(np.where(inbox, 0, sens)), g.shape),
-             np.unravel_index(np.argmax(sens), g.shape)]
-    picks = list(dict.fromkeys(tuple(int(i) for i in v) for v in picks))
-    if len(picks) < 3:
-        order = np.argsort(sens.ravel(order="C"))[::-1]
-        for f in order:
-            v = tuple(int(i) for i in np.unravel_index(f, g.shape))
-            if v not in picks:
-                picks.append(v)
-            if len(picks) == 3:
-                break
+    order = np.argsort(sens.ravel())[::-1][:3]
+    picks = [tuple(int(i) for i in np.unravel_index(f, g.shape)) for f in order]
     for v in picks:
         e = np.zeros(mua.shape, np.float32)
         e[v] = 1.0
-        f = fd(e, H_REL * float(mua[v]))
-        print(f"[gpu-config] blob128 voxel {v}: grad {g[v]:.6e} fd {f:.6e} ({(f - g[v]) / g[v]:+.2%})", flush=True)
-        assert abs(f - g[v]) <= 0.02 * abs(g[v]), (v, f, g[v])
+        check(f"voxel {v}", gs[(slice(None),) + v], fd(e, H_REL * float(mua[v])), False)
+    vo = tuple(int(i) for i in np.unravel_index(np.argmax(np.where(inbox, 0, sens)), g.shape))
+    e = np.zeros(mua.shape, np.float32)
+    e[vo] = 1.0
+    check(f"voxel {vo} (outside the box)", gs[(slice(None),) + vo], fd(e, H_REL * float(mua[vo])), True)
     # along the map itself: d/dt J(mua (1 + t)) at t = 0
-    f = fd(mua, H_REL)
-    dd = float(np.sum(g * mua))
-    print(f"[gpu-config] blob128 direction mua: grad {dd:.6e} fd {f:.6e} ({(f - dd) / dd:+.2%})", flush=True)
-    assert abs(f - dd) <= 0.02 * abs(dd), (f, dd)
+    check("direction mua", (gs * mua).sum(axis=(1, 2, 3)), fd(mua, H_REL), False)
 
 
 def test_blob128_heat_adjoint_3000_steps_matches_central_differences(api):

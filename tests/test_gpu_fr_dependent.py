@@ -55,3 +55,34 @@ def test_saturable_absorber_bins_follow_the_fluence_rate(api):
     lo, hi = M == 3, M == 3 + nB - 1  # lowest-mua and highest-mua sub-media
     assert lo.any() and hi.any()
     assert FR[lo].mean() > FR[hi].mean()  # strongly lit voxels are the most bleached
+
+
+def test_split_remaps_deposition_criteria_bounds(api):
+    """Criteria bounds on the split medium and on later media follow the
+    split (getOpticalMediaProperties.m:126-133): a lower bound on medium 2
+    stays at its first sub-medium, an upper bound on medium 3 moves by
+    nBins - 1.  The split run then counts scatterings in media 2..3 only,
+    like the oracle on the same split model."""
+    import oracle_lib as OL
+    from mcstats import compare_scalars
+
+    nB = 3
+    m = models.base(n=20, nph=2e4)
+    dc = m.MC.depositionCriteria
+    dc.minScatterings, dc.evaluateOnlyAtEndOfLife = 3, False
+    dc.minSubmediaIdx, dc.maxSubmediaIdx = 2, 3
+    r = api.fr_dependent_MCmatlab(m, 2, lambda FR: (20.0, 200.0, 0.9), nB, 1, seed=21)
+    rd = r.MC.depositionCriteria
+    assert (rd.minSubmediaIdx, rd.maxSubmediaIdx) == (2, 3 + nB - 1)
+    split = r.copy()
+    G, O = [], []
+    for s in range(6):
+        g = api.MCmatlab(split, 1, seed=300 + s).MC
+        G.append({"NFR": g.NFR, "NI_zneg": g.NI_zneg, "NI_zpos": g.NI_zpos})
+        O.append(OL.oracle_mc(split, seed=s))
+    bad = [row for row in compare_scalars(G, O, split) if not abs(row[1]) <= row[2]]
+    assert not bad, bad
+    t = api.t_dependent_heat  # the heat-side helper validates its medium count the same way
+    with pytest.raises(ValueError):
+        t(np.zeros((2, 2, 2), np.float32), np.float32(np.nan), {"A": [0], "E": [0]}, np.ones((2, 2, 2), np.uint8), [3.7], [0.0037],
+          (1e-3,) * 3, 1, lambda T: (3.7, 0.0037), 256, [(1, 1e-3, 0)])

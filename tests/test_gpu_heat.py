@@ -161,3 +161,19 @@ def test_gpu_heat_line_lengths(api, n, axis, bt):
     ref = OL.oracle_heat(T0, Om0, P, variant=OL.HEAT_FP64_DGADI)
     assert rel(T64, ref["T_f64"]) <= 1e-9
     assert rel(Om64, ref["Omega_f64"]) <= 1e-9
+
+
+@pytest.mark.parametrize("zrun", [8, 12, 16, 24, 32])
+@pytest.mark.parametrize("name", ["multi_tile", "multi_tile_insulated"])
+def test_gpu_heat_explicit_every_zrun_matches_oracle(api, name, zrun):
+    """Every launch shape the explicit plan can choose gives the oracle's
+    result (the shape only splits the grid into workgroup tiles)."""
+    T0, Om0, P = heat_case(**CASES[name])
+    with api.HeatPlan(T0, Om0, P, scheme="explicit") as h:
+        h.set_explicit_zrun(zrun)
+        assert h.explicit_zrun == zrun
+        h.advance(P["steps"], P["dt"], P["lightsOn"])
+        r = h.download(f64=True)
+    ref = OL.oracle_heat(T0, Om0, P, variant=OL.HEAT_FP64_EXPLICIT)
+    assert rel(r["T_f64"], ref["T_f64"]) <= 1e-9
+    assert rel(r["Omega_f64"], ref["Omega_f64"]) <= 1e-9

@@ -7,6 +7,11 @@ oracle in test_gpu_heat.py).
 - dJ/dT0 along a random direction, with FP64 initial states: 1e-6.
 - dJ/dQ along the uniform scaling of the heating rate (a single-precision
   input, so the differences are Richardson-extrapolated): 1e-4.
+- dJ/d dTdtperdeltaT (the conduction coefficients, through which the
+  thermal conductivities enter) along a random direction of the table
+  (Richardson-extrapolated central differences of the representable
+  single-precision perturbation): 1e-5; dJ/dTC through the chain rule and
+  the torch.autograd.Function against central differences in TC: 1e-3.
 - checkpoint segments of 5 steps over 23 steps (a partial last segment),
   fixed-temperature and insulating boundaries, custom target masks;
   lights-off schedules give dJ/dQ = 0.
@@ -161,3 +166,62 @@ def test_adjoints_in_threads_equal_one_at_a_time(api):
     for a, b in zip(alone, many):
         assert a["J"] == b["J"]
         assert np.array_equal(a["dJ_dQ"], b["dJ_dQ"]) and np.array_equal(a["dJ_dT0"], b["dJ_dT0"])
+
+
+@pytest.mark.parametrize("bt", [0, 1])
+def test_adjoint_dc_matches_central_differences(api, bt):
+    T0, Om0, P, segs = case(bt, seed=11)
+    r = api.heat_adjoint(T0, Om0, P, segments=segs, checkpointEvery=5, coefGrad=True)
+    g = r["dJ_dc"]
+    assert g.shape == (4, 4, 3) and np.all(np.isfinite(g)) and np.any(g != 0)
+    c = np.asarray(P["dTdtperdeltaT"], np.float64)
+    rng = np.random.default_rng(4)
+    d = rng.standard_normal(c.shape)
+
+    def D(h):
+        out = []
+        for sgn in (1, -1):
+            cp = (c * (1 + sgn * h * d)).astype(np.float32)
+            out.append((api.heat_adjoint(T0, Om0, dict(P, dTdtperdeltaT=cp), segments=segs, grads=False)["J"], cp.astype(np.float64)))
+        (jp, cp), (jm, cm) = out
+        return jp - jm, float(np.sum(g * (cp - cm)))
+
+    (dJ1, ad1), (dJ2, ad2) = D(4e-3), D(2e-3)
+    rich = (4 * (dJ2 / ad2) - dJ1 / ad1) / 3
+    assert abs(rich - 1) <= 1e-5, (dJ1 / ad1, dJ2 / ad2, rich)
+
+
+def test_adjoint_conductivity_torch_function(api):
+    """dJ/dTC per medium through heat_damage_function (torch) against central
+    differences of J in TC, and dJ/dQ, dJ/dT0 equal the plain adjoint's."""
+    torch = pytest.importorskip("torch")
+    from mcmatlab_amd.media import heat_coefficients
+
+    T0, Om0, P, segs = case(1, seed=13)
+    n = 16
+    d = 0.1 / n
+    VHC = np.array([1.2e-3, 3.7, 3.9, 3.8])
+    TC = np.array([2.6e-4, 0.0037, 0.0052, 0.0045])
+    setup = {"P": P, "VHC": VHC, "d": (d, d, d), "segments": segs, "checkpointEvery": 5}
+    HD = api.heat_damage_function()
+    q = torch.tensor(np.asarray(P["dTdt_abs"], np.float64), requires_grad=True)
+    t0 = torch.tensor(T0.astype(np.float64), requires_grad=True)
+    tc = torch.tensor(TC, requires_grad=True)
+    J = HD.apply(q, t0, tc, setup)
+    J.backward()
+    ref = api.heat_adjoint(T0, Om0, dict(P, dTdtperdeltaT=heat_coefficients(VHC, TC, d, d, d)), segments=segs, checkpointEvery=5,
+                           T_f64=T0.astype(np.float64))
+    assert abs(float(J) - ref["J"]) <= 1e-12 * ref["J"]
+    np.testing.assert_allclose(q.grad.numpy(), ref["dJ_dQ"], rtol=1e-12, atol=0)
+    np.testing.assert_allclose(t0.grad.numpy(), ref["dJ_dT0"], rtol=1e-12, atol=0)
+    for m in range(4):
+        h = 1e-3 * TC[m]
+        Js = []
+        for sgn in (1, -1):
+            tcm = TC.copy()
+            tcm[m] += sgn * h
+            Js.append(api.heat_adjoint(T0, Om0, dict(P, dTdtperdeltaT=heat_coefficients(VHC, tcm, d, d, d)), segments=segs, grads=False,
+                                       T_f64=T0.astype(np.float64))["J"])
+        fd = (Js[0] - Js[1]) / (2 * h)
+        gm = float(tc.grad[m])
+        assert abs(fd - gm) <= 1e-3 * abs(gm) + 1e-6 * float(np.max(np.abs(tc.grad.numpy()))), (m, fd, gm)

@@ -56,6 +56,27 @@ def test_graph_replays_equal_step_launches(api, precision, scheme, bt):
     assert np.all(np.isfinite(eager[-1]["sensorTemps"]))
 
 
+def run_no_sync(api, T0, Om0, P, schedule, graph_steps):
+    with api.HeatPlan(T0, Om0, P, graph_steps=graph_steps) as h:
+        for steps, dt, on in schedule:
+            h.advance(steps, dt, on)
+        return h.download(f64=True)
+
+
+def test_graph_cache_eviction_with_replays_in_flight(api):
+    """Six settings in a row with no download in between: the plan keeps four
+    captured graphs, so the fifth and sixth captures evict graphs whose
+    replays may still be queued on the stream; the eviction waits for them.
+    The result equals the step launches bit for bit."""
+    T0, Om0, Pon, _ = layered_tissue_heat(n=40, steps_on=100, steps_off=100)
+    dt0 = Pon["dt"]
+    schedule = [(40, dt0 * (1 + 0.1 * k), k % 2) for k in range(6)] + [(40, dt0, 1)]
+    a = run_no_sync(api, T0, Om0, Pon, schedule, 0)
+    b = run_no_sync(api, T0, Om0, Pon, schedule, 16)
+    for key in ("T_f64", "Omega_f64", "sensorTemps", "maxMediaTemps"):
+        assert np.array_equal(a[key], b[key]), key
+
+
 def test_graph_steps_validation(api):
     T0, Om0, Pon, _ = layered_tissue_heat(n=16, steps_on=10, steps_off=10)
     with api.HeatPlan(T0, Om0, Pon) as h:

@@ -60,3 +60,32 @@ def test_spectral_budget_fixture_matches_generator():
     lam = np.linspace(450, 950, 64)
     assert np.allclose(mua[1], 20 * np.exp(-(lam - 500) / 150))
     assert np.allclose(m.MC.mediaProperties["mus"][3], 470 * (lam / 500) ** -1.3)
+
+
+def test_conductivity_gradient_is_the_chain_rule_of_the_coefficients():
+    """conductivity_gradient(G) is the gradient of sum(G * c(TC)) with
+    c[a,b,d] = 2 TC_a TC_b / (TC_a + TC_b) / VHC_a / d^2 (central differences
+    in double precision), including the a == b entries (TC_eff = TC_a)."""
+    import numpy as np
+
+    from mcmatlab_amd.media import conductivity_gradient
+
+    rng = np.random.default_rng(3)
+    nM = 4
+    VHC = rng.uniform(1, 4, nM)
+    TC = rng.uniform(1e-3, 6e-3, nM)
+    dx, dy, dz = 1e-3, 2e-3, 1.5e-3
+    G = rng.standard_normal((nM, nM, 3))
+
+    def F(tc):
+        te = 2 * np.outer(tc, tc) / (tc[:, None] + tc[None, :])
+        c = np.stack([te / VHC[:, None] / dd ** 2 for dd in (dx, dy, dz)], axis=2)
+        return float(np.sum(G * c))
+
+    g = conductivity_gradient(G, VHC, TC, dx, dy, dz)
+    for m in range(nM):
+        h = 1e-6 * TC[m]
+        e = np.zeros(nM)
+        e[m] = h
+        fd = (F(TC + e) - F(TC - e)) / (2 * h)
+        assert abs(fd - g[m]) <= 1e-7 * abs(fd) + 1e-9 * np.abs(g).max(), (m, fd, g[m])

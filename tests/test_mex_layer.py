@@ -91,6 +91,28 @@ def test_heat_gateway_equals_api():
     np.testing.assert_array_equal(r["maxMediaTemps"].ravel(), mx.ravel())
 
 
+@built
+@pytest.mark.gpu
+def test_heat_gateway_precision_field():
+    """P.precision = 'single' selects the FP32 arithmetic, 'double' the FP64
+    default; anything else is an input error."""
+    from mcmatlab_amd import api
+    from test_gpu_heat import heat_case
+
+    T0, Om0, P = heat_case()
+    m = MexLib(HEAT_MEX)
+    r32 = run_heat_mex(m, T0, Om0, dict(P, precision="single"))
+    T32, Om32, _, _ = api.finiteElementHeatPropagator(T0, Om0, P, precision="fp32")
+    np.testing.assert_array_equal(r32["T"], T32)
+    np.testing.assert_array_equal(r32["Omega"], Om32)
+    r64 = run_heat_mex(m, T0, Om0, dict(P, precision="double"))
+    T64, _, _, _ = api.finiteElementHeatPropagator(T0, Om0, P)
+    np.testing.assert_array_equal(r64["T"], T64)
+    assert not np.array_equal(r32["T"], r64["T"])
+    with pytest.raises(RuntimeError, match="invalidInput"):
+        run_heat_mex(m, T0, Om0, dict(P, precision="half"))
+
+
 def _bin_mex(m, pv, n2):
     from mexdriver import mxSINGLE
 
