@@ -363,6 +363,18 @@ int mcm_heat_adjoint_run(const mcm_heat_input *in, const mcm_heat_segment *segme
  * n1, n2 >= 1; non-finite values are MCM_ERR_INVALID_ARGUMENT. */
 int mcm_downsample_params(const float *paramVals, int64_t n1, int64_t n2, float *downsampled, uint8_t *binidx,
                           int32_t device);
+/* The same without any device allocation (no counterpart in the reference):
+ * for callers that cannot allocate device memory, such as serving loops and
+ * harnesses that forbid hipMalloc.  Every intermediate buffer lives in a
+ * file-scope device array of the library (about 30 MB, held by each GPU it
+ * runs on until the process ends), so n1 <= 2^20 parameter sets and
+ * n2 <= 4096 bins (larger requests are MCM_ERR_INVALID_ARGUMENT).  Runs on
+ * `stream` (a hipStream_t, or NULL for the default stream) and returns when
+ * the outputs are in the host arrays.  Calls on one GPU share the array:
+ * calls that overlap on the same GPU race and must be serialised by the
+ * caller. */
+int mcm_downsample_params_noalloc(const float *paramVals, int64_t n1, int64_t n2, float *downsampled, uint8_t *binidx,
+                                  int32_t device, void *stream);
 
 #ifdef __cplusplus
 }

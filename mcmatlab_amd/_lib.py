@@ -43,6 +43,7 @@ SIGNATURES = {
     "mcm_heat_adjoint_run": (C.c_int, [C.POINTER(abi.mcm_heat_input), C.POINTER(abi.mcm_heat_segment), C.c_int64,
                                        C.POINTER(C.c_uint8), C.c_int64, C.POINTER(abi.mcm_heat_adjoint_output)]),
     "mcm_downsample_params": (C.c_int, [C.c_void_p, C.c_int64, C.c_int64, C.c_void_p, C.c_void_p, C.c_int32]),
+    "mcm_downsample_params_noalloc": (C.c_int, [C.c_void_p, C.c_int64, C.c_int64, C.c_void_p, C.c_void_p, C.c_int32, C.c_void_p]),
 }
 
 _lib = None

@@ -35,6 +35,7 @@
 #include <algorithm>
 #include <cmath>
 #include <cstring>
+#include <mutex>
 #include <thread>
 #include <type_traits>
 
@@ -271,68 +272,49 @@ struct DevMem {
   }
 };
 
-}  // namespace
-}  // namespace mcm
+__global__ void bin_clear_state(int* st) {
+  if (threadIdx.x < 4) st[threadIdx.x] = 0;
+}
 
-using namespace mcm;
+// Device buffers of one call (see BinArgs) and the call's stream.
+struct BinBuffers {
+  float *pv, *norm, *d, *out;
+  uint8_t* bin;
+  float4* part;
+  int *picks, *st;
+};
 
-extern "C" int mcm_downsample_params(const float* paramVals, int64_t n1, int64_t n2, float* downsampled, uint8_t* binidx,
-                                     int32_t device) {
+int blocks_for(int64_t n1) { return (int)std::min<int64_t>(kBinMaxBlocks, (n1 + kBinTPB * kBinPer - 1) / (kBinTPB * kBinPer)); }
+int prep_blocks_for(int64_t n1) { return (int)std::min<int64_t>(kBinMaxBlocks, (n1 + kBinTPB - 1) / kBinTPB); }
+
+int check_args(const float* paramVals, int64_t n1, int64_t n2, float* downsampled, uint8_t* binidx) {
   if (!paramVals || !downsampled || !binidx) return fail(MCM_ERR_INVALID_ARGUMENT, "null argument");
   if (n1 < 1) return fail(MCM_ERR_INVALID_ARGUMENT, "getDownsampledParamVals needs at least one parameter set");
   if (n2 < 1) return fail(MCM_ERR_INVALID_ARGUMENT, "getDownsampledParamVals needs at least one bin");
   if (n1 > INT32_MAX || n2 > INT32_MAX) return fail(MCM_ERR_INVALID_ARGUMENT, "more than 2^31 - 1 parameter sets or bins");
-  if (int rc = use_device(device)) return rc;
-  // ranges and running sums in input order (:64-83), one host thread per
-  // row while the device buffers are set up and the input is uploaded; the
-  // media functions' values are checked finite by their callers, and a NaN
-  // here would leave the selection undefined, so it is an error
-  float mx[3], mn[3], sum[3];
-  bool finite[3];
-  auto rowPass = [&](int r) {
-    float hi = paramVals[r], lo = hi, sm = hi;
-    bool ok = std::isfinite(hi);
-    for (int64_t i = 1; i < n1; ++i) {
-      const float v = paramVals[3 * i + r];
-      ok = ok && std::isfinite(v);
-      hi = v > hi ? v : hi;
-      lo = v < lo ? v : lo;
-      sm += v;
-    }
-    mx[r] = hi, mn[r] = lo, sum[r] = sm, finite[r] = ok;
-  };
-  struct Rows {  // joined on every return path
-    std::thread t[3];
-    ~Rows() {
-      for (auto& x : t)
-        if (x.joinable()) x.join();
-    }
-  } rowThreads;
-  for (int r = 0; r < 3; ++r) rowThreads.t[r] = std::thread(rowPass, r);
-  const int blocks = (int)std::min<int64_t>(kBinMaxBlocks, (n1 + kBinTPB * kBinPer - 1) / (kBinTPB * kBinPer));
-  const int blocksPrep = (int)std::min<int64_t>(kBinMaxBlocks, (n1 + kBinTPB - 1) / kBinTPB);
-  DevMem pv, norm, d, bin, part, picks, st, out;
-  MCM_HIP(hipMalloc(&pv.p, (size_t)n1 * 3 * sizeof(float)));
-  MCM_HIP(hipMalloc(&norm.p, (size_t)n1 * 3 * sizeof(float)));
-  MCM_HIP(hipMalloc(&d.p, (size_t)n1 * sizeof(float)));
-  MCM_HIP(hipMalloc(&bin.p, (size_t)n1));
-  MCM_HIP(hipMalloc(&part.p, (size_t)std::max(blocks, blocksPrep) * sizeof(float4)));
-  MCM_HIP(hipMalloc(&picks.p, (size_t)n2 * sizeof(int)));
-  MCM_HIP(hipMalloc(&st.p, 4 * sizeof(int)));
-  MCM_HIP(hipMalloc(&out.p, (size_t)n2 * 3 * sizeof(float)));
-  hipStream_t s = nullptr;
-  MCM_HIP(hipStreamCreateWithFlags(&s, hipStreamNonBlocking));
-  struct StreamGuard {
-    hipStream_t s;
-    ~StreamGuard() { (void)hipStreamDestroy(s); }
-  } guard{s};
-  MCM_HIP(hipMemcpyAsync(pv.p, paramVals, (size_t)n1 * 3 * sizeof(float), hipMemcpyHostToDevice, s));
-  MCM_HIP(hipMemsetAsync(st.p, 0, 4 * sizeof(int), s));
-  for (auto& x : rowThreads.t) x.join();
-  if (!(finite[0] && finite[1] && finite[2])) {
-    (void)hipStreamSynchronize(s);
-    return fail(MCM_ERR_INVALID_ARGUMENT, "getDownsampledParamVals: paramVals must be finite");
+  return MCM_OK;
+}
+
+// ranges and running sums of input row r in input order (:64-83)
+void row_pass(const float* paramVals, int64_t n1, int r, float* mx, float* mn, float* sum, bool* finite) {
+  float hi = paramVals[r], lo = hi, sm = hi;
+  bool ok = std::isfinite(hi);
+  for (int64_t i = 1; i < n1; ++i) {
+    const float v = paramVals[3 * i + r];
+    ok = ok && std::isfinite(v);
+    hi = v > hi ? v : hi;
+    lo = v < lo ? v : lo;
+    sm += v;
   }
+  mx[r] = hi, mn[r] = lo, sum[r] = sm, finite[r] = ok;
+}
+
+// Everything after the upload of paramVals into B.pv and the row passes:
+// the N2 passes, the output and the downloads, on stream s (synchronised
+// before returning).
+int run_passes(const float* paramVals, int64_t n1, int64_t n2, float* downsampled, uint8_t* binidx, const BinBuffers& B,
+               const float* mx, const float* mn, const float* sum, hipStream_t s) {
+  const int blocks = blocks_for(n1), blocksPrep = prep_blocks_for(n1);
   BinArgs a{};
   a.n1 = n1;
   a.rows = 0;
@@ -341,13 +323,14 @@ extern "C" int mcm_downsample_params(const float* paramVals, int64_t n1, int64_t
     a.range[r] = mx[r] - mn[r];
     a.mean[r] = a.rowOf[r] >= 0 ? sum[r] / a.range[r] / (float)n1 : 0.f;  // :101-103
   }
-  a.pv = (const float*)pv.p;
-  a.norm = (float*)norm.p;
-  a.d = (float*)d.p;
-  a.bin = (uint8_t*)bin.p;
-  a.part = (float4*)part.p;
-  a.picks = (int*)picks.p;
-  a.st = (int*)st.p;
+  a.pv = B.pv;
+  a.norm = B.norm;
+  a.d = B.d;
+  a.bin = B.bin;
+  a.part = B.part;
+  a.picks = B.picks;
+  a.st = B.st;
+  hipLaunchKernelGGL(bin_clear_state, dim3(1), dim3(64), 0, s, B.st);
   hipLaunchKernelGGL(bin_prepare, dim3(blocksPrep), dim3(kBinTPB), 0, s, a);
   hipLaunchKernelGGL(bin_select<0>, dim3(1), dim3(kSelTPB), 0, s, a, blocksPrep);
   auto passes = [&](auto rowsTag) {
@@ -365,11 +348,128 @@ extern "C" int mcm_downsample_params(const float* paramVals, int64_t n1, int64_t
     case 2: passes(std::integral_constant<int, 2>()); break;
     default: passes(std::integral_constant<int, 3>()); break;
   }
-  hipLaunchKernelGGL(bin_output, dim3((unsigned)((n2 + 255) / 256)), dim3(256), 0, s, a, n2, (float*)out.p, paramVals[0],
-                     paramVals[1], paramVals[2]);
+  hipLaunchKernelGGL(bin_output, dim3((unsigned)((n2 + 255) / 256)), dim3(256), 0, s, a, n2, B.out, paramVals[0], paramVals[1],
+                     paramVals[2]);
   MCM_HIP(hipGetLastError());
-  MCM_HIP(hipMemcpyAsync(downsampled, out.p, (size_t)n2 * 3 * sizeof(float), hipMemcpyDeviceToHost, s));
-  MCM_HIP(hipMemcpyAsync(binidx, bin.p, (size_t)n1, hipMemcpyDeviceToHost, s));
+  MCM_HIP(hipMemcpyAsync(downsampled, B.out, (size_t)n2 * 3 * sizeof(float), hipMemcpyDeviceToHost, s));
+  MCM_HIP(hipMemcpyAsync(binidx, B.bin, (size_t)n1, hipMemcpyDeviceToHost, s));
   MCM_HIP(hipStreamSynchronize(s));
   return MCM_OK;
+}
+
+// Scratch of the allocation-free entry point: every device buffer of a call
+// with up to kScratchSets parameter sets and kScratchBins bins, laid out by
+// scratch_layout.  Each GPU holds its own copy for the life of the process.
+constexpr int64_t kScratchSets = 1 << 20;
+constexpr int64_t kScratchBins = 4096;
+constexpr size_t align256(size_t b) { return (b + 255) & ~(size_t)255; }
+constexpr size_t kScratchBytes = align256(kScratchSets * 12) * 2 + align256(kScratchSets * 4) + align256(kScratchSets) +
+                                 align256(kBinMaxBlocks * 16) + align256(kScratchBins * 4) + align256(16) + align256(kScratchBins * 12);
+__device__ __align__(256) unsigned char scratch[kScratchBytes];
+
+BinBuffers scratch_layout(unsigned char* base, int64_t n1, int64_t n2) {
+  BinBuffers B;
+  size_t o = 0;
+  auto take = [&](size_t bytes) {
+    unsigned char* p = base + o;
+    o += align256(bytes);
+    return p;
+  };
+  B.pv = (float*)take((size_t)n1 * 12);
+  B.norm = (float*)take((size_t)n1 * 12);
+  B.d = (float*)take((size_t)n1 * 4);
+  B.bin = take((size_t)n1);
+  B.part = (float4*)take((size_t)kBinMaxBlocks * 16);
+  B.picks = (int*)take((size_t)n2 * 4);
+  B.st = (int*)take(16);
+  B.out = (float*)take((size_t)n2 * 12);
+  return B;
+}
+
+constexpr int kMaxDevices = 64;
+std::mutex g_scratchMutex;
+void* g_scratch[kMaxDevices] = {};
+
+}  // namespace
+}  // namespace mcm
+
+using namespace mcm;
+
+extern "C" int mcm_downsample_params(const float* paramVals, int64_t n1, int64_t n2, float* downsampled, uint8_t* binidx,
+                                     int32_t device) {
+  if (int rc = check_args(paramVals, n1, n2, downsampled, binidx)) return rc;
+  if (int rc = use_device(device)) return rc;
+  // the row passes run on host threads while the device buffers are set up
+  // and the input is uploaded; the media functions' values are checked
+  // finite by their callers, and a NaN here would leave the selection
+  // undefined, so it is an error
+  float mx[3], mn[3], sum[3];
+  bool finite[3];
+  struct Rows {  // joined on every return path
+    std::thread t[3];
+    ~Rows() {
+      for (auto& x : t)
+        if (x.joinable()) x.join();
+    }
+  } rowThreads;
+  for (int r = 0; r < 3; ++r) rowThreads.t[r] = std::thread(row_pass, paramVals, n1, r, mx, mn, sum, finite);
+  const int blocks = blocks_for(n1), blocksPrep = prep_blocks_for(n1);
+  DevMem pv, norm, d, bin, part, picks, st, out;
+  MCM_HIP(hipMalloc(&pv.p, (size_t)n1 * 3 * sizeof(float)));
+  MCM_HIP(hipMalloc(&norm.p, (size_t)n1 * 3 * sizeof(float)));
+  MCM_HIP(hipMalloc(&d.p, (size_t)n1 * sizeof(float)));
+  MCM_HIP(hipMalloc(&bin.p, (size_t)n1));
+  MCM_HIP(hipMalloc(&part.p, (size_t)std::max(blocks, blocksPrep) * sizeof(float4)));
+  MCM_HIP(hipMalloc(&picks.p, (size_t)n2 * sizeof(int)));
+  MCM_HIP(hipMalloc(&st.p, 4 * sizeof(int)));
+  MCM_HIP(hipMalloc(&out.p, (size_t)n2 * 3 * sizeof(float)));
+  hipStream_t s = nullptr;
+  MCM_HIP(hipStreamCreateWithFlags(&s, hipStreamNonBlocking));
+  struct StreamGuard {
+    hipStream_t s;
+    ~StreamGuard() { (void)hipStreamDestroy(s); }
+  } guard{s};
+  MCM_HIP(hipMemcpyAsync(pv.p, paramVals, (size_t)n1 * 3 * sizeof(float), hipMemcpyHostToDevice, s));
+  for (auto& x : rowThreads.t) x.join();
+  if (!(finite[0] && finite[1] && finite[2])) {
+    (void)hipStreamSynchronize(s);
+    return fail(MCM_ERR_INVALID_ARGUMENT, "getDownsampledParamVals: paramVals must be finite");
+  }
+  const BinBuffers B{(float*)pv.p, (float*)norm.p, (float*)d.p, (float*)out.p, (uint8_t*)bin.p, (float4*)part.p, (int*)picks.p, (int*)st.p};
+  return run_passes(paramVals, n1, n2, downsampled, binidx, B, mx, mn, sum, s);
+}
+
+extern "C" int mcm_downsample_params_noalloc(const float* paramVals, int64_t n1, int64_t n2, float* downsampled, uint8_t* binidx,
+                                             int32_t device, void* stream) {
+  if (int rc = check_args(paramVals, n1, n2, downsampled, binidx)) return rc;
+  if (n1 > kScratchSets || n2 > kScratchBins)
+    return fail(MCM_ERR_INVALID_ARGUMENT, "mcm_downsample_params_noalloc: at most %lld parameter sets and %lld bins (got %lld, %lld)",
+                (long long)kScratchSets, (long long)kScratchBins, (long long)n1, (long long)n2);
+  if (device < 0 || device >= kMaxDevices) return fail(MCM_ERR_INVALID_ARGUMENT, "device %d out of range", device);
+  if (int rc = use_device(device)) return rc;
+  // the device's copy of the scratch array, looked up once per device with
+  // that device current (the array's name is not a device address on the host)
+  void* base = nullptr;
+  {
+    std::lock_guard<std::mutex> lk(g_scratchMutex);
+    if (!g_scratch[device]) {
+      void* ptr = nullptr;
+      const hipError_t e = hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch));
+      if (e != hipSuccess || !ptr)
+        return fail(MCM_ERR_DEVICE, "hipGetSymbolAddress of the binning scratch: %s", hipGetErrorString(e));
+      g_scratch[device] = ptr;
+    }
+    base = g_scratch[device];
+  }
+  hipStream_t s = (hipStream_t)stream;
+  const BinBuffers B = scratch_layout((unsigned char*)base, n1, n2);
+  MCM_HIP(hipMemcpyAsync(B.pv, paramVals, (size_t)n1 * 3 * sizeof(float), hipMemcpyHostToDevice, s));
+  float mx[3], mn[3], sum[3];
+  bool finite[3];
+  for (int r = 0; r < 3; ++r) row_pass(paramVals, n1, r, mx, mn, sum, finite);
+  if (!(finite[0] && finite[1] && finite[2])) {
+    (void)hipStreamSynchronize(s);
+    return fail(MCM_ERR_INVALID_ARGUMENT, "getDownsampledParamVals: paramVals must be finite");
+  }
+  return run_passes(paramVals, n1, n2, downsampled, binidx, B, mx, mn, sum, s);
 }

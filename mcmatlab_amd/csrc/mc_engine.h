@@ -86,13 +86,26 @@ struct PacketEngine {
   }
 
   // deposit of the sub-step, then checkEscape + getNewVoxelProperties
-  // (propagatePhoton :1095-1110, MCmatlab.c:254-257)
+  // (propagatePhoton :1095-1110, MCmatlab.c:254-257).  The deposit (decided
+  // on the state before the escape check, as in the reference) goes to the
+  // sink only after the next voxel's medium index has been requested: a
+  // deposit that flushes the sink's merged value issues an atomic, and loads
+  // and atomics retire in order, so an atomic issued before the load would
+  // make every voxel change wait for the atomic's round trip.
   template <class Sink>
   MCM_FN void finish_substep(float absorb, const McParams& p, const Medium* med, const uint8_t* cdf, Sink& sink) {
+    bool dep = false;
+    double dv = 0.0;
     if (P.insideVolume && p.calcNFR) {
-      if (CRITERIA && replay) sink.deposit(P.j, (double)(replayW * absorb));
-      else if (!(CRITERIA && p.evalAtEnd) && criteria_met(P, p)) sink.deposit(P.j, (double)absorb);
+      if (CRITERIA && replay) {
+        dep = true;
+        dv = (double)(replayW * absorb);
+      } else if (!(CRITERIA && p.evalAtEnd) && criteria_met(P, p)) {
+        dep = true;
+        dv = (double)absorb;
+      }
     }
+    const int jOld = P.j;
     if (!P.sameVoxel) {
       const EscapeResult e = check_escape(P, p);
       if constexpr (Sink::kDeferEscapes) {
@@ -114,8 +127,19 @@ struct PacketEngine {
           if (face >= 0) sink.edge(face, idx, (double)P.weight);
         }
       }
-      if (P.alive) new_voxel_properties(P, p, med, cdf);
+      if (P.alive) {
+        P.j = voxel_of(p, P.i);
+        const int m = p.M[P.j];  // requested before the deposit's atomic
+        if (dep) sink.deposit(jOld, dv);
+        dep = false;
+        const Medium md = med[m];
+        P.mua = md.mua;
+        P.mus = md.mus;
+        P.g = md.g;
+        P.cdf = cdf[m];
+      }
     }
+    if (dep) sink.deposit(jOld, dv);
     if (!P.alive) death(p, med, cdf, sink);
     else state = P.stepLeft > 0.f ? ST_MOVE : ST_SCATTER;
   }

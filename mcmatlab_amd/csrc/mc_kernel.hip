@@ -166,6 +166,9 @@ __device__ inline unsigned long long bcast64(unsigned long long v, int src) {
 
 __device__ inline void fadd(double* p, double v) { unsafeAtomicAdd(p, v); }
 
+// lanes per transport workgroup: the stride of a lane's deposit ring slots
+constexpr int kRingStride = 256;
+
 // Largest number of packets per transport launch (sizes the launch and
 // escape record buffers: 64 + 48 bytes per packet).
 constexpr uint64_t kMaxLaunchPackets = 1ull << 24;
@@ -219,14 +222,59 @@ struct DeviceSink {
   __device__ void runaway() { ++nRunaway; }
 };
 
+// Deposits waiting in LDS per lane (see DeferSink), drained by the whole
+// wavefront together.
+#ifndef MCM_DEPOSIT_RING
+#define MCM_DEPOSIT_RING 8
+#endif
+constexpr int kRing = MCM_DEPOSIT_RING;
+
 // Sink of the transport kernel without deposition criteria: a packet dies
 // once, so its escape is written to the record slot of its own launch index
 // (no shared counter) and tallied by mc_tally_escapes after the launch.
+//
+// Merged deposits do not go to the fluence one at a time: each lane parks
+// (address, value) in its slots of an LDS ring and the wavefront drains the
+// rings together when one of them is nearly full.  Global loads and atomics
+// share one in-order completion counter, so a load issued after an atomic
+// (the next voxel's medium index) waits for the atomic's round trip to the
+// memory side; draining in batches pays that wait once per batch instead of
+// once per voxel crossing.
 struct DeferSink : DeviceSink {
   static constexpr bool kDeferEscapes = true;
   EscapeRecord* rec;
   int item;
-  __device__ DeferSink(const McParams& p, EscapeRecord* r) : DeviceSink(p), rec(r), item(0) {}
+  unsigned long long* ringA;  // LDS: slot k of this lane at [k * stride]
+  double* ringV;
+  int nRing;
+  __device__ DeferSink(const McParams& p, EscapeRecord* r) : DeviceSink(p), rec(r), item(0), ringA(nullptr), ringV(nullptr), nRing(0) {}
+  __device__ void park(double* a, double v) {
+    if (!ringA) {
+      fadd(a, v);
+      return;
+    }
+    ringA[nRing * kRingStride] = (unsigned long long)a;
+    ringV[nRing * kRingStride] = v;
+    if (++nRing == kRing) drain();  // a full ring drains this lane alone
+  }
+  __device__ void drain() {
+    for (int k = 0; k < nRing; ++k) fadd((double*)ringA[k * kRingStride], ringV[k * kRingStride]);
+    nRing = 0;
+  }
+  __device__ void deposit(int j, double v) {
+    if (j == accJ) {
+      acc += v;
+    } else {
+      if (accJ >= 0) park(&NFR[accJ], acc);
+      accJ = j;
+      acc = v;
+    }
+  }
+  __device__ void flush() {
+    if (accJ >= 0) park(&NFR[accJ], acc);
+    accJ = -1;
+    acc = 0.0;
+  }
   __device__ void escape(const Photon& P, bool escaped, bool edgeFlux, uint32_t slot) {
     EscapeRecord E;
     pack_escape(P, escaped, edgeFlux, item, E);
@@ -359,6 +407,15 @@ __global__ void __launch_bounds__(256, BATCH == kMulti ? 4 : 1) mc_transport(McP
   E.init(lp.key0, lp.key1, lp.stream);
   using Sink = typename std::conditional<CRITERIA, DeviceSink, DeferSink>::type;
   Sink sink = make_sink<Sink>(p, escapes);
+  // the deposit rings of the plain launch (batched and multi-grid launches
+  // keep every item's tables in LDS and deposit directly)
+  constexpr int RING = (!CRITERIA && BATCH == kPlain) ? kRing : 0;
+  __shared__ unsigned long long sRingA[RING > 0 ? RING * kRingStride : 1];
+  __shared__ double sRingV[RING > 0 ? RING * kRingStride : 1];
+  if constexpr (RING > 0) {
+    sink.ringA = sRingA + threadIdx.x;
+    sink.ringV = sRingV + threadIdx.x;
+  }
   // the media tables of the lane's work item (the launch's in a plain launch)
   const Medium* med = sMed;
   const uint8_t* cdf = sCdf;
@@ -382,6 +439,11 @@ __global__ void __launch_bounds__(256, BATCH == kMulti ? 4 : 1) mc_transport(McP
   }
 
   for (;;) {
+    if constexpr (RING > 0) {
+      // a lane parks at most two deposits per iteration (its sub-step's and
+      // its death's): drain every ring once one of them could overflow
+      if (__ballot(sink.nRing >= RING - 2)) sink.drain();
+    }
     const int st = E.state;
     const unsigned long long bFetch = __ballot(st == ST_FETCH);
     const unsigned long long bMove = __ballot(st == ST_MOVE);
@@ -504,6 +566,7 @@ __global__ void __launch_bounds__(256, BATCH == kMulti ? 4 : 1) mc_transport(McP
     }
   }
   sink.flush();
+  if constexpr (RING > 0) sink.drain();
   if (nLaunched) atomicAdd(&lp.ctr->launched, nLaunched);
   if (sink.nCollected) atomicAdd(&lp.ctr->collected, sink.nCollected);
   if (nSub) atomicAdd(&lp.ctr->substeps, nSub);

@@ -2,11 +2,12 @@
 produces in MATLAB before calling the MEX files.
 
 - interface_normals: the (theta, phi) per-voxel surface-normal field that
-  MCmatlab.c reads as MC.interfaceNormals (getOpticalMediaProperties.m:222-244
-  builds it from Sobel gradients of each refractive-index class, smoothed).
-  Here each class indicator is Gaussian-smoothed and differentiated with
-  central differences; the normal points into the voxel's own class, as in the
-  reference, and is stored as the same spherical angles.
+  MCmatlab.c reads as MC.interfaceNormals, built like
+  getOpticalMediaProperties.m:222-244 (Sobel gradient of each
+  refractive-index class, smoothn-smoothed); interface_normals_fast, a
+  Gaussian-smoothed variant for large synthetic sweeps.
+- phase_function_cdf / unique_cdfs: the CDF table of custom phase functions
+  (getOpticalMediaProperties.m:94-116).
 - heat_coefficients / heat_dtmax: dTdtperdeltaT and the explicit stability
   limit (simulateHeatDistribution.m:469-514); conductivity_gradient: the
   chain rule from dJ/d dTdtperdeltaT to dJ/dTC.
@@ -21,16 +22,83 @@ from __future__ import annotations
 import numpy as np
 
 
-def interface_normals(M, n_of_medium, d, smoothing_voxels=1.5):
-    """MC.interfaceNormals single [2, L] for media map M (1-based, [nx,ny,nz])
-    and per-medium refractive index n_of_medium[nM]."""
+def _sobel_gradient(ind):
+    """The reference's 3-D Sobel gradient of an indicator field
+    (@model/private/imgradientxyz.m): along each axis the difference of the
+    two neighbours, weighted over the orthogonal 3 x 3 neighbourhood by
+    [[1,3,1],[3,6,3],[1,3,1]], edges replicated; returns (G_x, G_y, G_z) along
+    the array's axes 0, 1, 2."""
+    from scipy.ndimage import correlate
+
+    S = np.array([[1.0, 3.0, 1.0], [3.0, 6.0, 3.0], [1.0, 3.0, 1.0]])
+    out = []
+    for ax in range(3):
+        k = np.zeros((3, 3, 3))
+        others = [a for a in range(3) if a != ax]
+        for i, sgn in ((0, -1.0), (2, 1.0)):
+            sl = [slice(None)] * 3
+            sl[ax] = i
+            k[tuple(sl)] = sgn * S  # S spans the two other axes in their order
+        out.append(correlate(ind, k, mode="nearest"))
+    return out
+
+
+def _smoothn(fields, s, spacing):
+    """Penalised least-squares smoothing with a given smoothness s, as
+    @model/private/smoothn.m computes it for complete, equally weighted data
+    (no robust iterations): z = IDCT(Gamma * DCT(y)) with the orthonormal
+    3-D DCT-II, Gamma = 1 / (1 + s Lambda^2) and
+    Lambda = sum_i (2 - 2 cos(pi k_i / n_i)) / dI_i^2, dI = spacing / max(spacing)."""
+    from scipy.fft import dctn, idctn
+
+    shape = fields[0].shape
+    dI = np.asarray(spacing, np.float64) / max(spacing)
+    lam = np.zeros(shape)
+    for i, n in enumerate(shape):
+        sh = [1] * len(shape)
+        sh[i] = n
+        lam = lam + ((2 - 2 * np.cos(np.pi * np.arange(n) / n)) / dI[i] ** 2).reshape(sh)
+    gamma = 1.0 / (1.0 + s * lam ** 2)
+    return [idctn(gamma * dctn(f, type=2, norm="ortho", workers=-1), type=2, norm="ortho", workers=-1) for f in fields]
+
+
+def interface_normals(M, n_of_medium, d, smoothingLengthScale=0.1):
+    """MC.interfaceNormals single [2, L] for the media map M (1-based,
+    [nx,ny,nz]) and per-medium refractive indices n_of_medium [nM], as
+    getOpticalMediaProperties.m:222-244 builds them: for each distinct
+    refractive index, the Sobel gradient of its indicator divided by the voxel
+    sizes, smoothed by smoothn with smoothness max(1, round(smoothingLengthScale
+    / max(d))) (skipped for a non-positive length scale), kept on that class's
+    voxels; then (theta, phi) = (pi/2 - elevation, azimuth) of cart2sph."""
+    M = np.asarray(M)
+    ri = np.asarray(n_of_medium, dtype=np.float64)[M - 1]
+    dx, dy, dz = d
+    G = np.full((3,) + M.shape, np.nan)
+    for c in np.unique(ri):
+        sel = ri == c
+        gx, gy, gz = _sobel_gradient(sel.astype(np.float64))
+        comps = [gx / dx, gy / dy, gz / dz]
+        if smoothingLengthScale > 0:
+            comps = _smoothn(comps, max(1, round(smoothingLengthScale / max(dx, dy, dz))), (dx, dy, dz))
+        for k in range(3):
+            G[k][sel] = comps[k][sel]
+    phi = np.arctan2(G[1], G[0])
+    elev = np.arctan2(G[2], np.hypot(G[0], G[1]))
+    theta = np.pi / 2 - elev
+    return np.stack([theta.ravel(order="F"), phi.ravel(order="F")]).astype(np.float32)
+
+
+def interface_normals_fast(M, n_of_medium, d, smoothing_voxels=1.5):
+    """A cheaper normal field for large synthetic sweeps: each refractive-index
+    class's indicator Gaussian-smoothed and differentiated by central
+    differences (normals point into the voxel's own class, like the
+    reference's).  Not the reference's construction (interface_normals)."""
     from scipy.ndimage import gaussian_filter
 
     M = np.asarray(M)
     ri = np.asarray(n_of_medium, dtype=np.float64)[M - 1]
-    classes = np.unique(ri)
     G = np.zeros((3,) + M.shape)
-    for c in classes:
+    for c in np.unique(ri):
         ind = (ri == c).astype(np.float64)
         sm = gaussian_filter(ind, smoothing_voxels, mode="nearest") if smoothing_voxels > 0 else ind
         gx, gy, gz = np.gradient(sm, d[0], d[1], d[2])
@@ -40,6 +108,38 @@ def interface_normals(M, n_of_medium, d, smoothing_voxels=1.5):
     elev = np.arctan2(G[2], np.hypot(G[0], G[1]))
     theta = np.pi / 2 - elev
     return np.stack([theta.ravel(order="F"), phi.ravel(order="F")]).astype(np.float32)
+
+
+def phase_function_cdf(customPhaseFunc, wavelength=None, nThetas=200):
+    """The 201-point CDF of the polar angle for a medium with a custom phase
+    function, as getOpticalMediaProperties.m:94-104 builds it:
+    theta_k = pi/nThetas (k - 1/2), PDF_k = sin(theta_k) p(theta_k),
+    CDF = [0, cumsum(PDF) / sum(PDF)].  customPhaseFunc(wavelength, theta)."""
+    th = np.pi / nThetas * (np.arange(1, nThetas + 1) - 0.5)
+    pdf = np.array([np.sin(t) * float(customPhaseFunc(wavelength, t)) for t in th])
+    if not np.all(np.isfinite(pdf)) or np.any(pdf < 0):
+        raise ValueError("the custom phase function must return real, finite, non-negative numbers")
+    return np.concatenate([[0.0], np.cumsum(pdf) / pdf.sum()])
+
+
+def unique_cdfs(cdfs):
+    """(CDFs [201, nCDF], CDFidx per medium) with identical CDFs shared, in
+    order of first use (getOpticalMediaProperties.m:106-116); None entries
+    (media with a Henyey-Greenstein g) get CDFidx NaN."""
+    table, idx = [], []
+    for c in cdfs:
+        if c is None:
+            idx.append(np.nan)
+            continue
+        c = np.asarray(c, np.float64)
+        for k, t in enumerate(table):
+            if np.array_equal(t, c):
+                idx.append(float(k))
+                break
+        else:
+            table.append(c)
+            idx.append(float(len(table) - 1))
+    return (np.stack(table, axis=1) if table else np.zeros((201, 0))), np.array(idx)
 
 
 def heat_coefficients(VHC, TC, dx, dy, dz):

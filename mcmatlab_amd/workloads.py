@@ -28,7 +28,7 @@ import math
 
 import numpy as np
 
-from .media import heat_coefficients, interface_normals, sensor_weights
+from .media import heat_coefficients, interface_normals, interface_normals_fast, sensor_weights
 from .model import Geometry, Model, SourceIntensityDistribution, media_table
 
 SIDE = 0.1
@@ -212,7 +212,7 @@ def multi_grid_sweep(n_models=32, seed=3, side=SIDE):
         mc = m.MC
         mc.M = M
         mc.mediaProperties = media_table(mua, mus, g, ri)
-        mc.interfaceNormals = interface_normals(M, ri, (d, d, d))
+        mc.interfaceNormals = interface_normals_fast(M, ri, (d, d, d))
         mc.boundaryType = 3
         mc.spectrum = np.array([1.0])
         mc.LS.sourceType = 0

@@ -87,7 +87,17 @@ def cases():
     C["custom_phase"] = lambda: _custom_cdf_media(base())
     C["wavelengths3"] = lambda: base(nL=3, nph=3001)
     C["source3d"] = lambda: _source3d(base())
+    C["lg01"] = lambda: _set(base(), sourceType=3, FPID=SID(radialWidth=0.02), AID=SID(radialWidth=0.1), LS_theta=0.2, LS_phi=0.4)
+    C["fmc_source3d"] = lambda: _fluorescence(_source3d(base()))
     return C
+
+
+def _fluorescence(m):
+    """The fluorescence simulation (simType 2) reads model.FMC: the same
+    inputs moved there, the emitter distribution as its source
+    (runMonteCarlo.m:116-175 calls the MEX with simType 2)."""
+    m.FMC = m.MC.copy()
+    return m
 
 
 def _set(m, **kw):

@@ -229,9 +229,13 @@ def test_blob128_gradients_match_central_differences(blob128):
     decision flips between mua + h and mua - h, so the per-seed differences
     measure that noise: the three voxels J is most sensitive to, and the
     direction of the map itself, agree within 2 % on the mean over the
-    seeds; the most sensitive voxel outside the target box (ten times
-    weaker, only the path-length term) agrees within 2 % or within four
-    standard errors of the per-seed differences, whichever is larger."""
+    seeds.  The most sensitive voxel outside the target box (ten times
+    weaker, only the path-length term) sits on the beam's ballistic path:
+    there h = 1e-3 mua changes a packet's FP32 weight by ~2e-6 relative,
+    which single precision resolves to only a few per cent -- the same
+    rounding for every packet on that deterministic path, so it does not
+    average out -- and its difference quotient takes h = 2e-2 mua; it agrees
+    within 2 % or four standard errors of the per-seed differences."""
     plan, mua, box = blob128
     runs = [plan.absorbed_power(box, BLOB_N, seed=s, muaMap=mua) for s in BLOB_SEEDS]
     gs = np.array([r["dJ_dmua"] for r in runs])
@@ -267,7 +271,7 @@ def test_blob128_gradients_match_central_differences(blob128):
     vo = tuple(int(i) for i in np.unravel_index(np.argmax(np.where(inbox, 0, sens)), g.shape))
     e = np.zeros(mua.shape, np.float32)
     e[vo] = 1.0
-    check(f"voxel {vo} (outside the box)", gs[(slice(None),) + vo], fd(e, H_REL * float(mua[vo])), True)
+    check(f"voxel {vo} (outside the box, h = 2e-2)", gs[(slice(None),) + vo], fd(e, 2e-2 * float(mua[vo])), True)
     # along the map itself: d/dt J(mua (1 + t)) at t = 0
     check("direction mua", (gs * mua).sum(axis=(1, 2, 3)), fd(mua, H_REL), False)
 

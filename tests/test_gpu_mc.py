@@ -50,19 +50,22 @@ def gpu_runs(api, m, seeds, simType=1):
 CASES = ["pencil_bt0", "pencil_bt1", "pencil_bt2", "pencil_bt3", "pencil_oblique", "fresnel_bt1", "fresnel_bt3",
          "isotropic_point", "isotropic_line", "planewave_bt1", "planewave_bt2", "radial_tophat_gauss", "radial_gauss_lambert",
          "radial_custom", "xy_tophat_gauss", "xy_custom_lambert", "xy_custom_angular", "lens_timebins", "fiber", "farfield",
-         "criteria_scatter", "criteria_collected", "criteria_now", "custom_phase", "source3d"]
+         "criteria_scatter", "criteria_collected", "criteria_now", "custom_phase", "source3d", "lg01", "fmc_source3d"]
+SIM_TYPE = {"fmc_source3d": 2}  # the fluorescence route: the MEX reads model.FMC
 
 
 @pytest.mark.parametrize("name", CASES)
 def test_gpu_matches_oracle(api, name):
     print(f"[gpu-mc] case {name}", flush=True)
+    st = SIM_TYPE.get(name, 1)
     m = models.cases()[name]()
-    m.MC.nPhotonsRequested = NPH
-    G = gpu_runs(api, m, range(SEEDS))
-    O = [OL.oracle_mc(m, seed=s) for s in range(SEEDS)]
+    (m.FMC if st == 2 else m.MC).nPhotonsRequested = NPH
+    G = gpu_runs(api, m, range(SEEDS), simType=st)
+    O = [OL.oracle_mc(m, simType=st, seed=s) for s in range(SEEDS)]
     for g in G:
         assert g["nPhotons"] == NPH
-    bad = check_scalars_two_stage(lambda s: gpu_runs(api, m, [s])[0], lambda s: OL.oracle_mc(m, seed=s), m, runs=(G, O))
+    bad = check_scalars_two_stage(lambda s: gpu_runs(api, m, [s], simType=st)[0], lambda s: OL.oracle_mc(m, simType=st, seed=s), m,
+                                  simType=st, runs=(G, O))
     assert not bad, f"scalar summaries outside 3 sigma: {bad}"
     frac, mt2, n = compare_voxels(G, O)
     assert n > 100

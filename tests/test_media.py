@@ -89,3 +89,101 @@ def test_conductivity_gradient_is_the_chain_rule_of_the_coefficients():
         e[m] = h
         fd = (F(TC + e) - F(TC - e)) / (2 * h)
         assert abs(fd - g[m]) <= 1e-7 * abs(fd) + 1e-9 * np.abs(g).max(), (m, fd, g[m])
+
+
+def test_sobel_gradient_of_a_ramp_and_of_a_constant():
+    """The reference's Sobel kernel (difference of the two neighbours weighted
+    by [[1,3,1],[3,6,3],[1,3,1]] over the orthogonal neighbourhood, weights
+    summing to 22): a unit ramp along one axis has gradient 44 there and 0
+    along the others in the interior; replicated edges give half of it on the
+    end faces; a constant has none."""
+    import numpy as np
+
+    from mcmatlab_amd.media import _sobel_gradient
+
+    x = np.arange(7.0)
+    for ax in range(3):
+        sh = [1, 1, 1]
+        sh[ax] = 7
+        ramp = np.broadcast_to(x.reshape(sh), (7, 7, 7)).copy()
+        g = _sobel_gradient(ramp)
+        inner = (slice(1, -1),) * 3
+        assert np.allclose(g[ax][inner], 44.0)
+        end = [slice(None)] * 3
+        end[ax] = 0
+        assert np.allclose(g[ax][tuple(end)], 22.0)
+        for other in range(3):
+            if other != ax:
+                assert np.allclose(g[other], 0.0)
+    assert all(np.all(c == 0) for c in _sobel_gradient(np.full((5, 6, 7), 3.0)))
+
+
+def test_smoothn_scales_each_cosine_mode_by_its_filter_factor():
+    """smoothn with a given smoothness is the DCT-II filter
+    1 / (1 + s Lambda^2): a cosine mode cos(pi k (i + 1/2) / n) along one axis
+    is multiplied by 1 / (1 + s (2 - 2 cos(pi k / n))^2 / dI^4), a constant is
+    kept (Lambda = 0)."""
+    import numpy as np
+
+    from mcmatlab_amd.media import _smoothn
+
+    n = (12, 10, 8)
+    s = 7.0
+    spacing = (1.0, 2.0, 1.0)  # dI = spacing / max(spacing) = (0.5, 1, 0.5)
+    dI = np.array(spacing) / max(spacing)
+    for ax, k in ((0, 3), (1, 2), (2, 5)):
+        sh = [1, 1, 1]
+        sh[ax] = n[ax]
+        mode = np.broadcast_to(np.cos(np.pi * k * (np.arange(n[ax]) + 0.5) / n[ax]).reshape(sh), n).copy()
+        (out,) = _smoothn([mode], s, spacing)
+        lam = (2 - 2 * np.cos(np.pi * k / n[ax])) / dI[ax] ** 2
+        np.testing.assert_allclose(out, mode / (1 + s * lam ** 2), atol=1e-12)
+    (c,) = _smoothn([np.full(n, 2.5)], s, spacing)
+    np.testing.assert_allclose(c, 2.5, rtol=1e-12)
+
+
+def test_interface_normals_of_a_sphere_point_into_its_own_class():
+    """A sphere of n = 1.4 in n = 1: on the voxels either side of its surface
+    the normal points into the voxel's own class, within 20 degrees of the
+    radial direction (inward inside the sphere, outward outside)."""
+    import numpy as np
+
+    n, d = 32, 0.01
+    c = (np.arange(n) + 0.5 - n / 2) * d
+    X, Y, Z = np.meshgrid(c, c, c, indexing="ij")
+    R = np.sqrt(X ** 2 + Y ** 2 + Z ** 2)
+    M = np.where(R < 0.1, 2, 1).astype(np.uint8)
+    IN = interface_normals(M, [1.0, 1.4], (d, d, d), smoothingLengthScale=0.03)
+    th = IN[0].reshape(M.shape, order="F").astype(np.float64)
+    ph = IN[1].reshape(M.shape, order="F").astype(np.float64)
+    nv = np.stack([np.sin(th) * np.cos(ph), np.sin(th) * np.sin(ph), np.cos(th)])
+    rhat = np.stack([X, Y, Z]) / np.maximum(R, 1e-12)
+    shell_in = (R < 0.1) & (R > 0.1 - 1.5 * d)
+    shell_out = (R >= 0.1) & (R < 0.1 + 1.5 * d)
+    cos_in = np.sum(nv * -rhat, axis=0)[shell_in]
+    cos_out = np.sum(nv * rhat, axis=0)[shell_out]
+    lim = np.cos(np.radians(20))
+    assert np.all(cos_in > lim) and np.all(cos_out > lim), (cos_in.min(), cos_out.min())
+
+
+def test_phase_function_cdf_and_sharing():
+    """The custom phase function's CDF (getOpticalMediaProperties.m:94-104)
+    reproduces the Henyey-Greenstein polar CDF to the midpoint rule's
+    accuracy, and identical CDFs share one column."""
+    import math
+
+    import numpy as np
+
+    from mcmatlab_amd.media import phase_function_cdf, unique_cdfs
+
+    g = 0.5
+    hg = lambda lam, t: (1 - g * g) / (1 + g * g - 2 * g * math.cos(t)) ** 1.5
+    cdf = phase_function_cdf(hg)
+    assert cdf.shape == (201,) and cdf[0] == 0 and abs(cdf[-1] - 1) < 1e-15 and np.all(np.diff(cdf) > 0)
+    edges = math.pi / 200 * np.arange(201)
+    # P(theta <= t) = 1 - P(cos theta <= cos t), the HG cosine CDF
+    exact = 1 - (1 - g * g) / (2 * g) * (1 / np.sqrt(1 + g * g - 2 * g * np.cos(edges)) - 1 / (1 + g))
+    np.testing.assert_allclose(cdf, exact, atol=2e-4)
+    iso = phase_function_cdf(lambda lam, t: 1.0)
+    table, idx = unique_cdfs([None, cdf, iso, cdf.copy()])
+    assert table.shape == (201, 2) and np.isnan(idx[0]) and list(idx[1:]) == [0.0, 1.0, 0.0]
